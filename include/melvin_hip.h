@@ -1,0 +1,186 @@
+/*
+ * melvin_hip.h -- C ABI of libmelvin_hip.so, the MI355X (gfx950) kernels
+ * behind the melvin_amd array backend.
+ *
+ * Every entry point takes raw device pointers and a HIP stream (passed as
+ * void*, a hipStream_t; NULL = the null stream), returns an int status
+ * (MLV_OK = 0) and never allocates device memory except in the *_create
+ * calls.  On failure mlv_last_error() returns a thread-local message.
+ * Calls are asynchronous on the stream.  Plans and batches are not
+ * thread-safe.
+ *
+ * Layouts (C order, z contiguous):
+ *   fully spectral  spectral (2nn+1, nm) complex, nn = (nx-1)/3, nm = (nz-1)/3;
+ *                   row r <-> kx = r (r <= nn) or r - (2nn+1); column <-> kz
+ *   Fourier-x/FD-z  spectral (nn, nz) complex, row <-> kx in [0, nn)
+ *   physical        (nx, nz) real
+ * "complex" is interleaved (re, im) of the plan's real type (float or
+ * double).  The Laplacian symbol (MLV_SPEC_LAP) and semi-implicit linear
+ * operators are always float64, as in the reference.
+ *
+ * Reference interfaces cited as path:line in JamieJQuinn/Melvin.py.
+ */
+#ifndef MELVIN_HIP_H
+#define MELVIN_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define MLV_OK 0
+#define MLV_ERR_ARG 1
+#define MLV_ERR_HIP 2
+#define MLV_ERR_UNSUPPORTED 3
+
+/* discretisation (Parameters.py:22, 57-61) */
+#define MLV_FULLY_SPECTRAL 0
+#define MLV_FOURIER_X_FD_Z 1
+
+/* real type of the plan (Parameters.py:84-89) */
+#define MLV_F32 0
+#define MLV_F64 1
+
+/* spectral operators (SpatialDifferentiator.py:50-74, Variable.py:111-113,
+ * LaplacianSolver.py:58-68) */
+#define MLV_SPEC_DDX 0
+#define MLV_SPEC_DDZ 1
+#define MLV_SPEC_D2DX2 2
+#define MLV_SPEC_D2DZ2 3
+#define MLV_SPEC_LAP 4
+#define MLV_SPEC_SNABLA2 5
+#define MLV_SPEC_POISSON 6
+
+/* physical finite differences (SpatialDifferentiator.py:76-185) */
+#define MLV_FD_DDX 0
+#define MLV_FD_DDZ 1
+#define MLV_FD_D2DZ2 2
+
+/* time integration (Integrator.py:53-63, 78-83) */
+#define MLV_SCHEME_SEMI_IMPLICIT 0
+#define MLV_SCHEME_EXPLICIT 1
+
+/* FP64 reductions (Integrator.py:38-39 xp.max, utility.py:58 xp.sum,
+ * double_diffusive_convection.py:22 xp.mean) */
+#define MLV_RED_MAX 0
+#define MLV_RED_SUM 1
+#define MLV_RED_SUMSQ2 2 /* sum(a*a + b*b) */
+#define MLV_RED_DOT 3    /* sum(a*b) */
+
+/* fused step problems: the example loop bodies */
+#define MLV_PROB_VORTICITY 0 /* taylor_green_vortex.py:87-93, vortex_pairs.py:122-128 */
+#define MLV_PROB_VORT_SCALAR 1 /* kelvin_helmholtz_instability.py:117-129, ink enabled */
+#define MLV_PROB_DDC 2       /* double_diffusive_convection.py:102-124 */
+#define MLV_PROB_MHD 3       /* resistive_tearing_instability.py:127-146 */
+
+typedef struct mlv_plan mlv_plan;
+typedef struct mlv_batch mlv_batch;
+
+/* Thread-local message of the last failure; never NULL. */
+const char* mlv_last_error(void);
+/* ABI version, bumped on any signature change. */
+int mlv_abi_version(void);
+
+/* ---------------------------------------------------------------------
+ * Plans: one grid, discretisation and precision.  Replaces the state the
+ * reference keeps in Parameters (Parameters.py:63-92), ArrayFactory
+ * (ArrayFactory.py:8-26), SpatialDifferentiator (SpatialDifferentiator.py:
+ * 8-48) and the FD-z LaplacianSolver factorisation (LaplacianSolver.py:
+ * 21-56).  nx must be a power of two in [8, 4096]; nz too when fully
+ * spectral, else any nz >= 3.
+ * ------------------------------------------------------------------- */
+int mlv_plan_create(mlv_plan** plan, int nx, int nz, double lx, double lz, int discretisation,
+                    int dtype, int fd_order);
+int mlv_plan_destroy(mlv_plan* plan);
+/* nn, nm (nm = nz for FD-z), spectral rows and columns */
+int mlv_plan_info(const mlv_plan* plan, int* nn, int* nm, int* spec_rows, int* spec_cols);
+
+/* SpectralTransformer.to_physical (SpectralTransformer.py:33-61, 90-150) */
+int mlv_to_physical(mlv_plan* plan, const void* spec, void* phys, void* stream);
+/* SpectralTransformer.to_spectral (SpectralTransformer.py:63-88, 152-199) */
+int mlv_to_spectral(mlv_plan* plan, const void* phys, void* spec, void* stream);
+/* SpatialDifferentiator sddx/sddz/sd2dx2/sd2dz2/calc_lap, Variable.snabla2
+ * and the fully spectral LaplacianSolver.solve; `in` is spectral, `out` is
+ * spectral (complex) or, for MLV_SPEC_LAP, float64 with `in` ignored.  In
+ * FD-z, MLV_SPEC_D2DZ2 / MLV_SPEC_SNABLA2 use the FD z-derivative
+ * (SpatialDifferentiator.py:35-40). */
+int mlv_spec_op(mlv_plan* plan, int kind, const void* in, void* out, void* stream);
+/* SpatialDifferentiator.pddx/pddz/pd2dz2 on a rows x cols array of real
+ * (is_complex = 0) or complex values (SpatialDifferentiator.py:76-185). */
+int mlv_fd_op(mlv_plan* plan, int kind, int is_complex, int rows, int cols, const void* in,
+              void* out, void* stream);
+/* pddx(ux*f) + pddz(uz*f), physical (Variable.py:125-127) */
+int mlv_flux_divergence(mlv_plan* plan, const void* ux, const void* uz, const void* f, void* out,
+                        void* stream);
+/* LaplacianSolver.solve: fully spectral divide (LaplacianSolver.py:58-68) or
+ * the per-mode tridiagonal FD-z solve (LaplacianSolver.py:70-78), FP64
+ * internally. */
+int mlv_poisson(mlv_plan* plan, const void* rhs, void* out, void* stream);
+/* Integrator.integrate for one variable: store nothing new, combine the
+ * history ring (order 2 or 4, current slot `curr`) with Adams-Bashforth and
+ * apply the semi-implicit (lin = float64 linear operator) or explicit
+ * (lin = complex diffusion term, added into the current history slot)
+ * update in place (Integrator.py:5-18, 53-63). */
+int mlv_update(mlv_plan* plan, int scheme, int order, double alpha, double dt, const void* lin,
+               void* var, void* hist, int curr, void* stream);
+/* Integrator.predictor (moulton = 0: AB2/AB4, Integrator.py:5-18) or
+ * Integrator.corrector (moulton = 1: AM2/AM4, Integrator.py:20-33) of the
+ * history ring into out. */
+int mlv_combine(mlv_plan* plan, int moulton, int order, double dt, const void* hist, int curr,
+                void* out, void* stream);
+/* calc_velocity_from_vorticity's spectral part (utility.py:62-79):
+ * psi = solve(-w); fully spectral also ux_spec = -ddz(psi) and
+ * uz_spec = ddx(psi); FD-z gives psi (tridiagonal solve) and uz_spec, and
+ * ux_spec must be NULL (ux is the physical -pddz of psi there).  Outputs
+ * may be NULL except psi. */
+int mlv_velocity(mlv_plan* plan, const void* w, void* psi, void* ux_spec, void* uz_spec,
+                 void* stream);
+/* FP64 reduction of n values of type dtype (b used by SUMSQ2/DOT); writes
+ * scale * result to out_f64 (device).  plan may be NULL. */
+int mlv_reduce(mlv_plan* plan, int op, int dtype, const void* a, const void* b, size_t n,
+               double scale, void* out_f64, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Fused steps over a table of jobs (independent simulations).  One call
+ * advances every job one full loop body of its problem: the velocity solve
+ * (utility.py:62-79), every advected field's vec_dot_nabla (Variable.py:
+ * 119-128) and source terms, and Integrator.integrate (Integrator.py:5-18,
+ * 58-63), i.e. the bodies cited at MLV_PROB_*.  Fully spectral only.
+ * ------------------------------------------------------------------- */
+typedef struct mlv_job_desc {
+  int problem;         /* MLV_PROB_* */
+  int nx, nz;          /* powers of two in [8, 4096] */
+  double lx, lz;
+  int fd_order;        /* 2 or 4 */
+  int ab_order;        /* 2 or 4 */
+  double alpha;        /* semi-implicit weight (Parameters.py:13) */
+  double dt;
+  double coeff[4];     /* diffusion coefficient per field equation */
+  double param[4];     /* DDC: Pr, R0 */
+  int nfield;
+  void* state[2][4];   /* ping-pong spectral state per field */
+  void* hist[4];       /* AB ring per field: ab_order x spectral */
+  void* work;          /* mlv_job_work_bytes() bytes */
+  void* psi;           /* optional spectral psi of the step, or NULL */
+  void* phys[8];       /* optional physical outputs (problem order), or NULL */
+  int curr;            /* ring slot the next step writes */
+  int parity;          /* state buffer holding the current state */
+} mlv_job_desc;
+
+int mlv_job_work_bytes(int problem, int nfield, int nx, int nz, int dtype, size_t* bytes);
+int mlv_batch_create(mlv_batch** batch, int dtype, const mlv_job_desc* jobs, int njobs);
+int mlv_batch_destroy(mlv_batch* batch);
+/* Run nsteps steps; write_phys = 1 also writes the physical fields of the
+ * last of them.  Each step reads dt from the job table. */
+int mlv_batch_step(mlv_batch* batch, int nsteps, int write_phys, void* stream);
+/* Set one job's dt for the following steps (stream-ordered). */
+int mlv_batch_set_dt(mlv_batch* batch, int job, double dt, void* stream);
+/* Steps completed since creation (host count). */
+long long mlv_batch_steps(const mlv_batch* batch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MELVIN_HIP_H */

@@ -1,0 +1,88 @@
+"""Poisson solve on the GPU (mirror of melvin/LaplacianSolver.py).
+
+Fully spectral: ``rhs / lap`` with ``lap[0,0]`` taken as 1
+(LaplacianSolver.py:58-68).  Fourier-x / FD-z: for every x mode the
+tridiagonal second-difference matrix in z with identity wall rows
+(21-52), solved by a batched FP64 Thomas sweep whose elimination factors
+are precomputed at plan creation (70-78).  ``laps[n]`` rebuilds mode n's
+matrix on the host for inspection (it supports ``@`` like the reference's
+scipy matrices).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse
+
+from . import _lib, hip
+from ._arrays import as_input, finish, output
+from ._plans import plan_for, ptr
+from .BasisFunctions import BasisFunctions, calc_diff_factor
+
+
+class _FdmOperator:
+    """Mode n's FD-z Laplacian (identity wall rows) as a host CSR matrix."""
+
+    def __init__(self, params, n, ddx_factor):
+        nz, dz = params.nz, params.dz
+        ones = np.ones(nz)
+        m = scipy.sparse.dia_matrix(
+            (np.array([1.0 / dz ** 2 * ones,
+                       -((n * np.abs(ddx_factor)) ** 2 + 2.0 / dz ** 2) * ones,
+                       1.0 / dz ** 2 * ones]), np.array([-1, 0, 1])),
+            shape=(nz, nz), dtype=np.complex128).tolil()
+        m[0, 0], m[0, 1] = 1.0, 0.0
+        m[-1, -1], m[-1, -2] = 1.0, 0.0
+        self._m = m.tocsr()
+
+    def toarray(self):
+        return self._m.toarray()
+
+    def __matmul__(self, vec):
+        if hasattr(vec, "is_cuda"):
+            return hip.asarray(self._m @ hip.asnumpy(vec))
+        return self._m @ vec
+
+
+class _FdmOperators:
+    def __init__(self, params, ddx_factor):
+        self._p = params
+        self._f = ddx_factor
+
+    def __len__(self):
+        return self._p.nn
+
+    def __getitem__(self, n):
+        if not 0 <= n < self._p.nn:
+            raise IndexError(n)
+        return _FdmOperator(self._p, n, self._f)
+
+
+class LaplacianSolver:
+    def __init__(self, params, xp=hip, basis_fns=None, spatial_diff=None,
+                 array_factory=None):
+        self._params = params
+        self._xp = hip
+        self._array_factory = array_factory
+        self._plan = plan_for(params)
+        if params.is_fully_spectral():
+            if spatial_diff is not None and basis_fns is not None:
+                self.lap = spatial_diff.calc_lap(basis_fns)
+            self.solve = self._solve_fully_spectral
+        else:
+            # x is always the complex-exponential basis here
+            self.laps = _FdmOperators(params, calc_diff_factor(
+                BasisFunctions.COMPLEX_EXP, params.lx))
+            self.solve = self._solve_fdm
+
+    def _solve(self, rhs, out):
+        pl = self._plan
+        src = as_input(rhs, pl.cplx, (pl.rows, pl.cols))
+        dst, ret = output(out, pl.cplx, (pl.rows, pl.cols))
+        _lib.call("mlv_poisson", pl.handle, ptr(src), ptr(dst), _lib.stream_handle())
+        return finish(dst, ret)
+
+    def _solve_fully_spectral(self, rhs, out=None):
+        return self._solve(rhs, out)
+
+    def _solve_fdm(self, rhs, out=None):
+        return self._solve(rhs, out)

@@ -1,0 +1,815 @@
+// extern "C" entry points of libmelvin_hip.so (declared in
+// include/melvin_hip.h): plans, operator-level calls, fused job batches.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/melvin_hip.h"
+#include "fused.hpp"
+#include "ops.hpp"
+#include "transforms.hpp"
+
+using namespace mlv;
+
+namespace {
+
+thread_local std::string g_err = "";
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  g_err = std::string(what) + ": " + hipGetErrorString(e);
+  return MLV_ERR_HIP;
+}
+
+#define HIP_TRY(call, what)                    \
+  do {                                         \
+    hipError_t e_ = (call);                    \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+int ilog2_exact(int n) {
+  if (n <= 0 || (n & (n - 1))) return -1;
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
+
+// FP64-tabulated twiddles exp(-2 pi i k / NMAX), built once per device.
+struct Twiddles {
+  void* f32 = nullptr;
+  void* f64 = nullptr;
+};
+std::mutex g_tw_mu;
+std::map<int, Twiddles> g_tw;
+
+int get_twiddles(int device, int dtype, const void** out) {
+  std::lock_guard<std::mutex> lk(g_tw_mu);
+  Twiddles& t = g_tw[device];
+  if (!t.f64) {
+    std::vector<double> d(2 * NMAX);
+    std::vector<float> f(2 * NMAX);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int k = 0; k < NMAX; ++k) {
+      const long double a = 2.0L * pi * (long double)k / (long double)NMAX;
+      const long double c = cosl(a), s = -sinl(a);
+      d[2 * k] = (double)c;
+      d[2 * k + 1] = (double)s;
+      f[2 * k] = (float)c;
+      f[2 * k + 1] = (float)s;
+    }
+    HIP_TRY(hipMalloc(&t.f64, d.size() * sizeof(double)), "hipMalloc twiddles");
+    HIP_TRY(hipMalloc(&t.f32, f.size() * sizeof(float)), "hipMalloc twiddles");
+    HIP_TRY(hipMemcpy(t.f64, d.data(), d.size() * sizeof(double), hipMemcpyHostToDevice),
+            "upload twiddles");
+    HIP_TRY(hipMemcpy(t.f32, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice),
+            "upload twiddles");
+  }
+  *out = dtype == MLV_F64 ? t.f64 : t.f32;
+  return MLV_OK;
+}
+
+size_t max_lds_per_block(int device) {
+  int v = 0;
+  if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+    return 65536;
+  return (size_t)v;
+}
+
+size_t cplx_bytes(int dtype) { return dtype == MLV_F64 ? 16 : 8; }
+
+// Reduction partials for calls without a plan: one buffer per device,
+// allocated on first use.
+std::mutex g_red_mu;
+std::map<int, double*> g_red;
+
+int global_partials(int device, double** out) {
+  std::lock_guard<std::mutex> lk(g_red_mu);
+  double*& p = g_red[device];
+  if (!p) HIP_TRY(hipMalloc((void**)&p, reduce_partials_bytes()), "hipMalloc partials");
+  *out = p;
+  return MLV_OK;
+}
+
+}  // namespace
+
+struct mlv_plan {
+  int device;
+  int nx, nz, lognx, lognz, nn, nm, rows, cols;
+  int fully, dtype, fd_order;
+  double lx, lz, dx, dz, fx, fz, fx2, fz2;
+  const void* tw;
+  void* work;     // 2-D: (nx, nm) complex intermediate; FD-z: Thomas d' (nn, nz) complex128
+  double* cp;     // FD-z Thomas factors (nn, nz)
+  double* rden;
+  double lower;
+  double* red_part;
+};
+
+extern "C" {
+
+const char* mlv_last_error(void) { return g_err.c_str(); }
+int mlv_abi_version(void) { return 1; }
+
+int mlv_plan_create(mlv_plan** out, int nx, int nz, double lx, double lz, int disc, int dtype,
+                    int fd_order) {
+  if (!out) return fail(MLV_ERR_ARG, "plan pointer is NULL");
+  *out = nullptr;
+  if (dtype != MLV_F32 && dtype != MLV_F64) return fail(MLV_ERR_ARG, "dtype must be F32 or F64");
+  if (fd_order != 2 && fd_order != 4) return fail(MLV_ERR_ARG, "fd_order must be 2 or 4");
+  if (disc != MLV_FULLY_SPECTRAL && disc != MLV_FOURIER_X_FD_Z)
+    return fail(MLV_ERR_ARG, "unknown discretisation");
+  if (!(lx > 0) || !(lz > 0)) return fail(MLV_ERR_ARG, "domain lengths must be positive");
+  const int lnx = ilog2_exact(nx);
+  if (lnx < 3 || lnx > 12) return fail(MLV_ERR_UNSUPPORTED, "nx must be a power of two in [8, 4096]");
+  int lnz = -1;
+  if (disc == MLV_FULLY_SPECTRAL) {
+    lnz = ilog2_exact(nz);
+    if (lnz < 3 || lnz > 12)
+      return fail(MLV_ERR_UNSUPPORTED, "nz must be a power of two in [8, 4096]");
+  } else if (nz < 5) {
+    return fail(MLV_ERR_UNSUPPORTED, "FD-z needs nz >= 5");
+  }
+  mlv_plan* p = new mlv_plan();
+  std::memset(p, 0, sizeof(*p));
+  HIP_TRY(hipGetDevice(&p->device), "hipGetDevice");
+  p->nx = nx;
+  p->nz = nz;
+  p->lognx = lnx;
+  p->lognz = lnz;
+  p->fully = disc == MLV_FULLY_SPECTRAL;
+  p->dtype = dtype;
+  p->fd_order = fd_order;
+  p->lx = lx;
+  p->lz = lz;
+  p->nn = (nx - 1) / 3;
+  if (p->fully) {
+    p->nm = (nz - 1) / 3;
+    p->rows = 2 * p->nn + 1;
+    p->cols = p->nm;
+  } else {
+    p->nm = nz;
+    p->rows = p->nn;
+    p->cols = nz;
+  }
+  // Parameters.py:81-82 and BasisFunctions.py:26-59, in the same operation order
+  p->dx = lx / nx;
+  p->dz = lz / nz;
+  const double two_pi = 2.0 * M_PI;
+  p->fx = two_pi / lx;
+  p->fz = two_pi / lz;
+  p->fx2 = -(two_pi * two_pi) / (lx * lx);
+  p->fz2 = -(two_pi * two_pi) / (lz * lz);
+  int rc = get_twiddles(p->device, dtype, &p->tw);
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  if (p->fully) {
+    HIP_TRY(hipMalloc(&p->work, (size_t)nx * p->nm * cplx_bytes(dtype)), "hipMalloc workspace");
+  } else {
+    // Thomas factors of LaplacianSolver.py:21-52, built as a NumPy sweep would
+    const int nn = p->nn;
+    std::vector<double> cp((size_t)nn * nz), rden((size_t)nn * nz);
+    const double inv_dz2 = 1.0 / (p->dz * p->dz);
+    const double ddx = two_pi / lx;
+    for (int n = 0; n < nn; ++n) {
+      const double diag = -((n * ddx) * (n * ddx) + 2.0 / (p->dz * p->dz));
+      double cprev = 0.0;
+      for (int j = 0; j < nz; ++j) {
+        const double dj = (j == 0 || j == nz - 1) ? 1.0 : diag;
+        const double lj = (j == 0 || j == nz - 1) ? 0.0 : inv_dz2;
+        const double uj = (j == 0 || j == nz - 1) ? 0.0 : inv_dz2;
+        const double den = j == 0 ? dj : dj - lj * cprev;
+        const double c = uj / den;
+        cp[(size_t)n * nz + j] = c;
+        rden[(size_t)n * nz + j] = 1.0 / den;
+        cprev = c;
+      }
+    }
+    p->lower = inv_dz2;
+    HIP_TRY(hipMalloc(&p->work, (size_t)nn * nz * 16), "hipMalloc workspace");
+    HIP_TRY(hipMalloc((void**)&p->cp, cp.size() * sizeof(double)), "hipMalloc thomas");
+    HIP_TRY(hipMalloc((void**)&p->rden, rden.size() * sizeof(double)), "hipMalloc thomas");
+    HIP_TRY(hipMemcpy(p->cp, cp.data(), cp.size() * sizeof(double), hipMemcpyHostToDevice),
+            "upload thomas");
+    HIP_TRY(hipMemcpy(p->rden, rden.data(), rden.size() * sizeof(double), hipMemcpyHostToDevice),
+            "upload thomas");
+  }
+  HIP_TRY(hipMalloc((void**)&p->red_part, reduce_partials_bytes()), "hipMalloc partials");
+  const size_t lds_need = dtype == MLV_F64 ? fft_lds_bytes<double>(lnx) : fft_lds_bytes<float>(lnx);
+  if (lds_need > max_lds_per_block(p->device)) {
+    mlv_plan_destroy(p);
+    return fail(MLV_ERR_UNSUPPORTED, "transform does not fit the LDS of this device");
+  }
+  *out = p;
+  return MLV_OK;
+}
+
+int mlv_plan_destroy(mlv_plan* p) {
+  if (!p) return MLV_OK;
+  if (p->work) (void)hipFree(p->work);
+  if (p->cp) (void)hipFree(p->cp);
+  if (p->rden) (void)hipFree(p->rden);
+  if (p->red_part) (void)hipFree(p->red_part);
+  delete p;
+  return MLV_OK;
+}
+
+int mlv_plan_info(const mlv_plan* p, int* nn, int* nm, int* rows, int* cols) {
+  if (!p) return fail(MLV_ERR_ARG, "plan is NULL");
+  if (nn) *nn = p->nn;
+  if (nm) *nm = p->nm;
+  if (rows) *rows = p->rows;
+  if (cols) *cols = p->cols;
+  return MLV_OK;
+}
+
+#define CHECK_PLAN(p) \
+  if (!(p)) return fail(MLV_ERR_ARG, "plan is NULL")
+#define CHECK_PTR(x) \
+  if (!(x)) return fail(MLV_ERR_ARG, #x " is NULL")
+#define DISPATCH(p, CALL_F32, CALL_F64) ((p)->dtype == MLV_F64 ? (CALL_F64) : (CALL_F32))
+
+int mlv_to_physical(mlv_plan* p, const void* spec, void* phys, void* stream) {
+  CHECK_PLAN(p);
+  CHECK_PTR(spec);
+  CHECK_PTR(phys);
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
+  if (p->fully) {
+    e = DISPATCH(p, launch_xinv<float>(p->lognx, spec, p->work, p->nn, p->nm, p->tw, s),
+                 launch_xinv<double>(p->lognx, spec, p->work, p->nn, p->nm, p->tw, s));
+    if (e != hipSuccess) return hip_fail(e, "to_physical x pass");
+    e = DISPATCH(p, launch_zc2r<float>(p->lognz, p->work, phys, p->nx, p->nm, p->tw, s),
+                 launch_zc2r<double>(p->lognz, p->work, phys, p->nx, p->nm, p->tw, s));
+    if (e != hipSuccess) return hip_fail(e, "to_physical z pass");
+  } else {
+    e = DISPATCH(p, launch_xc2r_cols<float>(p->lognx, spec, phys, p->nn, p->nz, p->tw, s),
+                 launch_xc2r_cols<double>(p->lognx, spec, phys, p->nn, p->nz, p->tw, s));
+    if (e != hipSuccess) return hip_fail(e, "to_physical x pass");
+  }
+  return MLV_OK;
+}
+
+int mlv_to_spectral(mlv_plan* p, const void* phys, void* spec, void* stream) {
+  CHECK_PLAN(p);
+  CHECK_PTR(spec);
+  CHECK_PTR(phys);
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
+  if (p->fully) {
+    const double scale = 1.0 / ((double)p->nx * (double)p->nz);
+    e = DISPATCH(p, launch_zr2c<float>(p->lognz, phys, p->work, p->nx, p->nm, p->tw, s),
+                 launch_zr2c<double>(p->lognz, phys, p->work, p->nx, p->nm, p->tw, s));
+    if (e != hipSuccess) return hip_fail(e, "to_spectral z pass");
+    e = DISPATCH(p, launch_xfwd<float>(p->lognx, p->work, spec, p->nn, p->nm, scale, p->tw, s),
+                 launch_xfwd<double>(p->lognx, p->work, spec, p->nn, p->nm, scale, p->tw, s));
+    if (e != hipSuccess) return hip_fail(e, "to_spectral x pass");
+  } else {
+    const double scale = 1.0 / (double)p->nx;
+    e = DISPATCH(p,
+                 launch_xr2c_cols<float>(p->lognx, phys, spec, p->nn, p->nz, scale, p->tw, s),
+                 launch_xr2c_cols<double>(p->lognx, phys, spec, p->nn, p->nz, scale, p->tw, s));
+    if (e != hipSuccess) return hip_fail(e, "to_spectral x pass");
+  }
+  return MLV_OK;
+}
+
+int mlv_spec_op(mlv_plan* p, int kind, const void* in, void* out, void* stream) {
+  CHECK_PLAN(p);
+  CHECK_PTR(out);
+  if (kind != MLV_SPEC_LAP) CHECK_PTR(in);
+  hipStream_t s = (hipStream_t)stream;
+  if (kind < MLV_SPEC_DDX || kind > MLV_SPEC_POISSON) return fail(MLV_ERR_ARG, "unknown spectral op");
+  if (!p->fully) {
+    // FD-z: z derivatives are finite differences on the complex rows
+    // (SpatialDifferentiator.py:35-40); the FD-z Laplacian has no symbol.
+    if (kind == MLV_SPEC_DDZ || kind == MLV_SPEC_POISSON)
+      return fail(MLV_ERR_UNSUPPORTED,
+                  "operator not defined for Fourier-x/FD-z (use mlv_fd_op / mlv_poisson)");
+    if (kind == MLV_SPEC_LAP) {
+      // the FDM basis has a zero second-derivative factor (BasisFunctions.py:40-43)
+      hipError_t e = DISPATCH(p,
+                              launch_spec<float>(in, out, p->rows, p->cols, p->nn, 0, kind, p->fx,
+                                                 p->fz, p->fx2, 0.0, s),
+                              launch_spec<double>(in, out, p->rows, p->cols, p->nn, 0, kind, p->fx,
+                                                  p->fz, p->fx2, 0.0, s));
+      if (e != hipSuccess) return hip_fail(e, "lap");
+      return MLV_OK;
+    }
+    if (kind == MLV_SPEC_D2DZ2)
+      return mlv_fd_op(p, MLV_FD_D2DZ2, 1, p->rows, p->cols, in, out, stream);
+    if (kind == MLV_SPEC_SNABLA2) {
+      // sd2dx2 + FD d2/dz2: out = d2z(in); then out += d2x(in)
+      int rc = mlv_fd_op(p, MLV_FD_D2DZ2, 1, p->rows, p->cols, in, out, stream);
+      if (rc) return rc;
+      hipError_t e =
+          DISPATCH(p,
+                   launch_spec<float>(in, out, p->rows, p->cols, p->nn, 0, kSpecAddD2dx2, p->fx,
+                                      p->fz, p->fx2, p->fz2, s),
+                   launch_spec<double>(in, out, p->rows, p->cols, p->nn, 0, kSpecAddD2dx2, p->fx,
+                                       p->fz, p->fx2, p->fz2, s));
+      if (e != hipSuccess) return hip_fail(e, "snabla2");
+      return MLV_OK;
+    }
+  }
+  hipError_t e = DISPATCH(p,
+                          launch_spec<float>(in, out, p->rows, p->cols, p->nn, p->fully, kind,
+                                             p->fx, p->fz, p->fx2, p->fz2, s),
+                          launch_spec<double>(in, out, p->rows, p->cols, p->nn, p->fully, kind,
+                                              p->fx, p->fz, p->fx2, p->fz2, s));
+  if (e != hipSuccess) return hip_fail(e, "spectral op");
+  return MLV_OK;
+}
+
+int mlv_fd_op(mlv_plan* p, int kind, int is_complex, int rows, int cols, const void* in, void* out,
+              void* stream) {
+  CHECK_PLAN(p);
+  CHECK_PTR(in);
+  CHECK_PTR(out);
+  if (rows <= 0 || cols <= 0) return fail(MLV_ERR_ARG, "bad array shape");
+  if (kind < MLV_FD_DDX || kind > MLV_FD_D2DZ2) return fail(MLV_ERR_ARG, "unknown FD op");
+  // the stencils wrap at most once, so the differentiated axis needs 5 points
+  if ((kind == MLV_FD_DDX ? rows : cols) < 5)
+    return fail(MLV_ERR_ARG, "FD axis shorter than 5 points");
+  int periodic = 0;
+  double h = p->dz;
+  if (kind == MLV_FD_DDX) {
+    periodic = 1;  // x is spectral in every supported discretisation
+    h = p->dx;
+  } else if (kind == MLV_FD_DDZ) {
+    periodic = p->fully;
+  }
+  hipError_t e = DISPATCH(
+      p,
+      launch_fd<float>(in, out, rows, cols, is_complex ? 2 : 1, kind, p->fd_order, periodic, h,
+                       (hipStream_t)stream),
+      launch_fd<double>(in, out, rows, cols, is_complex ? 2 : 1, kind, p->fd_order, periodic, h,
+                        (hipStream_t)stream));
+  if (e != hipSuccess) return hip_fail(e, "fd op");
+  return MLV_OK;
+}
+
+int mlv_flux_divergence(mlv_plan* p, const void* ux, const void* uz, const void* f, void* out,
+                        void* stream) {
+  CHECK_PLAN(p);
+  CHECK_PTR(ux);
+  CHECK_PTR(uz);
+  CHECK_PTR(f);
+  CHECK_PTR(out);
+  hipError_t e = DISPATCH(
+      p,
+      launch_flux<float>(ux, uz, f, out, p->nx, p->nz, p->fd_order, 1, p->fully, p->dx, p->dz,
+                         (hipStream_t)stream),
+      launch_flux<double>(ux, uz, f, out, p->nx, p->nz, p->fd_order, 1, p->fully, p->dx, p->dz,
+                          (hipStream_t)stream));
+  if (e != hipSuccess) return hip_fail(e, "flux divergence");
+  return MLV_OK;
+}
+
+int mlv_poisson(mlv_plan* p, const void* rhs, void* out, void* stream) {
+  CHECK_PLAN(p);
+  CHECK_PTR(rhs);
+  CHECK_PTR(out);
+  if (p->fully) return mlv_spec_op(p, MLV_SPEC_POISSON, rhs, out, stream);
+  hipError_t e = DISPATCH(p,
+                          launch_thomas<float>(rhs, out, p->work, p->cp, p->rden, p->lower, 1.0,
+                                               p->nn, p->nz, (hipStream_t)stream),
+                          launch_thomas<double>(rhs, out, p->work, p->cp, p->rden, p->lower, 1.0,
+                                                p->nn, p->nz, (hipStream_t)stream));
+  if (e != hipSuccess) return hip_fail(e, "poisson");
+  return MLV_OK;
+}
+
+int mlv_update(mlv_plan* p, int scheme, int order, double alpha, double dt, const void* lin,
+               void* var, void* hist, int curr, void* stream) {
+  CHECK_PLAN(p);
+  CHECK_PTR(lin);
+  CHECK_PTR(var);
+  CHECK_PTR(hist);
+  if (order != 2 && order != 4) return fail(MLV_ERR_ARG, "integrator order must be 2 or 4");
+  if (curr < 0 || curr >= order) return fail(MLV_ERR_ARG, "curr out of range");
+  if (scheme != MLV_SCHEME_SEMI_IMPLICIT && scheme != MLV_SCHEME_EXPLICIT)
+    return fail(MLV_ERR_ARG, "unknown scheme");
+  const size_t n = (size_t)p->rows * p->cols;
+  const double c1 = (1 - alpha) * dt, c2 = alpha * dt;
+  const double h = order == 2 ? dt / 2 : dt / 24;
+  hipError_t e = DISPATCH(
+      p, launch_update<float>(var, hist, lin, n, order, curr, scheme, c1, c2, h, (hipStream_t)stream),
+      launch_update<double>(var, hist, lin, n, order, curr, scheme, c1, c2, h, (hipStream_t)stream));
+  if (e != hipSuccess) return hip_fail(e, "update");
+  return MLV_OK;
+}
+
+int mlv_reduce(mlv_plan* p, int op, int dtype, const void* a, const void* b, size_t n,
+               double scale, void* out, void* stream) {
+  CHECK_PTR(a);
+  CHECK_PTR(out);
+  if (op < MLV_RED_MAX || op > MLV_RED_DOT) return fail(MLV_ERR_ARG, "unknown reduction");
+  if ((op == MLV_RED_SUMSQ2 || op == MLV_RED_DOT) && !b) return fail(MLV_ERR_ARG, "b is NULL");
+  if (n == 0) return fail(MLV_ERR_ARG, "empty reduction");
+  double* part = p ? p->red_part : nullptr;
+  if (!part) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+    int rc = global_partials(dev, &part);
+    if (rc) return rc;
+  }
+  hipError_t e =
+      dtype == MLV_F64
+          ? launch_reduce<double>(a, b, n, op, scale, part, (double*)out, (hipStream_t)stream)
+          : launch_reduce<float>(a, b, n, op, scale, part, (double*)out, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "reduce");
+  return MLV_OK;
+}
+
+int mlv_combine(mlv_plan* p, int moulton, int order, double dt, const void* hist, int curr,
+                void* out, void* stream) {
+  CHECK_PLAN(p);
+  CHECK_PTR(hist);
+  CHECK_PTR(out);
+  if (order != 2 && order != 4) return fail(MLV_ERR_ARG, "integrator order must be 2 or 4");
+  if (curr < 0 || curr >= order) return fail(MLV_ERR_ARG, "curr out of range");
+  const size_t n = (size_t)p->rows * p->cols;
+  const double h = order == 2 ? dt / 2 : dt / 24;
+  hipError_t e = DISPATCH(
+      p, launch_combine<float>(hist, out, n, order, curr, moulton ? 1 : 0, h, (hipStream_t)stream),
+      launch_combine<double>(hist, out, n, order, curr, moulton ? 1 : 0, h, (hipStream_t)stream));
+  if (e != hipSuccess) return hip_fail(e, "combine");
+  return MLV_OK;
+}
+
+int mlv_velocity(mlv_plan* p, const void* w, void* psi, void* ux_spec, void* uz_spec,
+                 void* stream) {
+  CHECK_PLAN(p);
+  CHECK_PTR(w);
+  CHECK_PTR(psi);
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e;
+  if (p->fully) {
+    e = DISPATCH(p,
+                 launch_velocity<float>(w, psi, ux_spec, uz_spec, p->rows, p->cols, p->nn, p->fx,
+                                        p->fz, p->fx2, p->fz2, s),
+                 launch_velocity<double>(w, psi, ux_spec, uz_spec, p->rows, p->cols, p->nn, p->fx,
+                                         p->fz, p->fx2, p->fz2, s));
+    if (e != hipSuccess) return hip_fail(e, "velocity");
+    return MLV_OK;
+  }
+  if (ux_spec) return fail(MLV_ERR_ARG, "FD-z: ux comes from the physical z-derivative of psi");
+  e = DISPATCH(p,
+               launch_thomas<float>(w, psi, p->work, p->cp, p->rden, p->lower, -1.0, p->nn, p->nz, s),
+               launch_thomas<double>(w, psi, p->work, p->cp, p->rden, p->lower, -1.0, p->nn, p->nz, s));
+  if (e != hipSuccess) return hip_fail(e, "velocity poisson");
+  if (uz_spec) return mlv_spec_op(p, MLV_SPEC_DDX, psi, uz_spec, stream);
+  return MLV_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches of fused jobs
+// ---------------------------------------------------------------------------
+namespace {
+
+struct BatchHost {
+  int dtype = 0, njobs = 0, device = 0, nstage = 1;
+  void* jobs_dev = nullptr;
+  BatchLaunch bl{};
+  long long* step_dev = nullptr;
+  long long steps = 0;
+  const void* tw = nullptr;
+  std::vector<void*> allocs;
+};
+
+int problem_fields(int prob) {
+  switch (prob) {
+    case MLV_PROB_VORTICITY:
+      return -1;  // 1..3, given by nfield
+    case MLV_PROB_VORT_SCALAR:
+      return 2;
+    case MLV_PROB_DDC:
+      return 3;
+    case MLV_PROB_MHD:
+      return 2;
+    default:
+      return 0;
+  }
+}
+
+template <typename T>
+int build_job(const mlv_job_desc& d, JobDev<T>& J, int j) {
+  char msg[160];
+  auto bad = [&](const char* what) {
+    std::snprintf(msg, sizeof(msg), "job %d: %s", j, what);
+    return fail(MLV_ERR_ARG, msg);
+  };
+  std::memset(&J, 0, sizeof(J));
+  const int lnx = ilog2_exact(d.nx), lnz = ilog2_exact(d.nz);
+  if (lnx < 3 || lnx > 12 || lnz < 3 || lnz > 12) {
+    std::snprintf(msg, sizeof(msg), "job %d: nx, nz must be powers of two in [8, 4096]", j);
+    return fail(MLV_ERR_UNSUPPORTED, msg);
+  }
+  int nf = problem_fields(d.problem);
+  if (nf == 0) return bad("unknown problem");
+  if (nf < 0) {
+    if (d.nfield < 1 || d.nfield > 3) return bad("vorticity problem needs 1..3 fields");
+    nf = d.nfield;
+  } else if (d.nfield != nf) {
+    return bad("nfield does not match the problem");
+  }
+  if (d.ab_order != 2 && d.ab_order != 4) return bad("ab_order must be 2 or 4");
+  if (d.fd_order != 2 && d.fd_order != 4) return bad("fd_order must be 2 or 4");
+  if (!(d.lx > 0) || !(d.lz > 0) || !(d.dt > 0)) return bad("lengths and dt must be positive");
+  if (d.curr < 0 || d.curr >= d.ab_order || (d.parity & ~1)) return bad("curr/parity out of range");
+  if (!d.work) return bad("work is NULL");
+  for (int f = 0; f < nf; ++f)
+    if (!d.state[0][f] || !d.state[1][f] || !d.hist[f]) return bad("state/hist pointer is NULL");
+  J.problem = d.problem;
+  J.nx = d.nx;
+  J.nz = d.nz;
+  J.lognx = lnx;
+  J.lognz = lnz;
+  J.nn = (d.nx - 1) / 3;
+  J.nm = (d.nz - 1) / 3;
+  J.rows = 2 * J.nn + 1;
+  J.q = d.ab_order;
+  J.curr0 = d.curr;
+  J.par0 = d.parity;
+  J.fd_order = d.fd_order;
+  const double two_pi = 2.0 * M_PI;
+  J.fx = two_pi / d.lx;
+  J.fz = two_pi / d.lz;
+  J.fx2 = -(two_pi * two_pi) / (d.lx * d.lx);
+  J.fz2 = -(two_pi * two_pi) / (d.lz * d.lz);
+  J.dx = d.lx / d.nx;
+  J.dz = d.lz / d.nz;
+  J.scale = 1.0 / ((double)d.nx * (double)d.nz);
+  J.alpha = d.alpha;
+  J.dt = d.dt;
+  J.nfield = nf;
+  for (int i = 0; i < 4; ++i) {
+    J.coeff[i] = d.coeff[i];
+    J.param[i] = d.param[i];
+  }
+  for (int b = 0; b < 2; ++b)
+    for (int f = 0; f < nf; ++f) J.state[b][f] = (cx<T>*)d.state[b][f];
+  for (int f = 0; f < nf; ++f) J.hist[f] = (cx<T>*)d.hist[f];
+  J.work = (cx<T>*)d.work;
+  J.psi = (cx<T>*)d.psi;
+  for (int i = 0; i < kMaxPhys; ++i) J.phys[i] = (T*)d.phys[i];
+  for (int s = 0; s < 2; ++s)
+    for (int i = 0; i < kMaxPhys; ++i) J.st[s].phys_ch[i] = -1;
+  auto addA = [&](StageDev& S, int src, int op, int ch, int from_new) {
+    S.a[S.nA++] = XinvOp{src, op, ch, from_new};
+  };
+  if (d.problem == MLV_PROB_MHD) {
+    J.nstage = 2;
+    StageDev& s0 = J.st[0];
+    addA(s0, 0, XOP_ID, 0, 0);
+    addA(s0, 1, XOP_ID, 1, 0);
+    addA(s0, 0, XOP_UX, 2, 0);
+    addA(s0, 0, XOP_UZ, 3, 0);
+    addA(s0, 1, XOP_UX, 4, 0);
+    addA(s0, 1, XOP_UZ, 5, 0);
+    s0.bvar = BV_MHD1;
+    s0.neq = 1;
+    s0.eq_field[0] = 0;
+    for (int i = 0; i < 6; ++i)
+      if (J.phys[i]) s0.phys_ch[i] = i;
+    StageDev& s1 = J.st[1];
+    addA(s1, 0, XOP_ID, 0, 1);
+    s1.bvar = BV_MHD2;
+    s1.neq = 1;
+    s1.eq_field[0] = 1;
+  } else {
+    J.nstage = 1;
+    StageDev& s0 = J.st[0];
+    for (int f = 0; f < nf; ++f) addA(s0, f, XOP_ID, f, 0);
+    addA(s0, 0, XOP_UX, nf, 0);
+    addA(s0, 0, XOP_UZ, nf + 1, 0);
+    s0.bvar = BV_VORT1 + (nf - 1);
+    s0.neq = nf;
+    for (int f = 0; f < nf; ++f) s0.eq_field[f] = f;
+    for (int i = 0; i < nf + 2; ++i)
+      if (J.phys[i]) s0.phys_ch[i] = i;
+  }
+  J.pbase = bvar_nin(J.st[0].bvar);
+  return MLV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mlv_job_work_bytes(int problem, int nfield, int nx, int nz, int dtype, size_t* bytes) {
+  if (!bytes) return fail(MLV_ERR_ARG, "bytes is NULL");
+  if (nx < 8 || nz < 8) return fail(MLV_ERR_ARG, "grid too small");
+  int nf = problem_fields(problem);
+  if (nf == 0) return fail(MLV_ERR_ARG, "unknown problem");
+  if (nf < 0) {
+    if (nfield < 1 || nfield > 3) return fail(MLV_ERR_ARG, "vorticity problem needs 1..3 fields");
+    nf = nfield;
+  }
+  const int bv = problem == MLV_PROB_MHD ? BV_MHD1 : BV_VORT1 + (nf - 1);
+  const int nm = (nz - 1) / 3;
+  *bytes = (size_t)(bvar_nin(bv) + 2 * bvar_neq(bv)) * nx * nm * cplx_bytes(dtype);
+  return MLV_OK;
+}
+
+int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int njobs) {
+  if (!out || !jobs || njobs <= 0) return fail(MLV_ERR_ARG, "bad batch arguments");
+  *out = nullptr;
+  if (dtype != MLV_F32 && dtype != MLV_F64) return fail(MLV_ERR_ARG, "dtype must be F32 or F64");
+  BatchHost* b = new BatchHost();
+  b->dtype = dtype;
+  b->njobs = njobs;
+  HIP_TRY(hipGetDevice(&b->device), "hipGetDevice");
+  int rc = get_twiddles(b->device, dtype, &b->tw);
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  const size_t lds_max = max_lds_per_block(b->device);
+  const size_t jsz = dtype == MLV_F64 ? sizeof(JobDev<double>) : sizeof(JobDev<float>);
+  std::vector<char> host(jsz * njobs);
+  std::vector<int> pA[2], pB[2][BV_COUNT], pC[2];
+  for (int s = 0; s < 2; ++s) {
+    pA[s].assign(njobs + 1, 0);
+    pC[s].assign(njobs + 1, 0);
+    for (int v = 0; v < BV_COUNT; ++v) pB[s][v].assign(njobs + 1, 0);
+  }
+  BatchLaunch& bl = b->bl;
+  std::memset(&bl, 0, sizeof(bl));
+  bl.njobs = njobs;
+  for (int j = 0; j < njobs; ++j) {
+    int nstage, lognx, lognz, nm, nx;
+    int nA[2], neq[2], bvar[2];
+    if (dtype == MLV_F64) {
+      JobDev<double>& J = reinterpret_cast<JobDev<double>*>(host.data())[j];
+      rc = build_job<double>(jobs[j], J, j);
+      if (rc) break;
+      nstage = J.nstage;
+      lognx = J.lognx;
+      lognz = J.lognz;
+      nm = J.nm;
+      nx = J.nx;
+      for (int s = 0; s < nstage; ++s) {
+        nA[s] = J.st[s].nA;
+        neq[s] = J.st[s].neq;
+        bvar[s] = J.st[s].bvar;
+      }
+    } else {
+      JobDev<float>& J = reinterpret_cast<JobDev<float>*>(host.data())[j];
+      rc = build_job<float>(jobs[j], J, j);
+      if (rc) break;
+      nstage = J.nstage;
+      lognx = J.lognx;
+      lognz = J.lognz;
+      nm = J.nm;
+      nx = J.nx;
+      for (int s = 0; s < nstage; ++s) {
+        nA[s] = J.st[s].nA;
+        neq[s] = J.st[s].neq;
+        bvar[s] = J.st[s].bvar;
+      }
+    }
+    const int slx = dtype == MLV_F64 ? slots_for<double>(lognx) : slots_for<float>(lognx);
+    const int slz = dtype == MLV_F64 ? slots_for<double>(lognz) : slots_for<float>(lognz);
+    const size_t ldsx = dtype == MLV_F64 ? fft_lds_bytes<double>(lognx) : fft_lds_bytes<float>(lognx);
+    const size_t ldsz = dtype == MLV_F64 ? fft_lds_bytes<double>(lognz) : fft_lds_bytes<float>(lognz);
+    if (ldsx > lds_max || ldsz > lds_max || slx < 2) {
+      rc = fail(MLV_ERR_UNSUPPORTED, "job " + std::to_string(j) + ": grid too large for LDS");
+      break;
+    }
+    if (nstage > b->nstage) b->nstage = nstage;
+    for (int s = 0; s < 2; ++s) {
+      int a = 0, c = 0, bb = 0, bv = 0;
+      if (s < nstage) {
+        const int nin_v = bvar_nin(bvar[s]), neq_v = bvar_neq(bvar[s]);
+        if (!bfits(nin_v, neq_v, slz)) {
+          rc = fail(MLV_ERR_UNSUPPORTED,
+                    "job " + std::to_string(j) + ": problem does not fit the z-pass LDS at this nz");
+          break;
+        }
+        const int rb = brows(nin_v, neq_v, slz);
+        a = nA[s] * ((nm + slx - 1) / slx);
+        c = neq[s] * ((nm + slx / 2 - 1) / (slx / 2));
+        bb = (nx + rb - 1) / rb;
+        bv = bvar[s];
+        bl.ldsB[bv] = std::max(bl.ldsB[bv], ldsz);
+      }
+      pA[s][j + 1] = pA[s][j] + a;
+      pC[s][j + 1] = pC[s][j] + c;
+      for (int v = 0; v < BV_COUNT; ++v) pB[s][v][j + 1] = pB[s][v][j] + (v == bv ? bb : 0);
+    }
+    if (rc) break;
+    bl.ldsA = std::max(bl.ldsA, ldsx);
+    bl.ldsC = std::max(bl.ldsC, ldsx);
+  }
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  auto dalloc = [&](size_t bytes, void** p) -> int {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc batch");
+    b->allocs.push_back(*p);
+    return MLV_OK;
+  };
+  auto upload_pref = [&](const std::vector<int>& v, int** dst) -> int {
+    int r = dalloc(v.size() * sizeof(int), (void**)dst);
+    if (r) return r;
+    HIP_TRY(hipMemcpy(*dst, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice), "upload");
+    return MLV_OK;
+  };
+  rc = dalloc(host.size(), &b->jobs_dev);
+  if (!rc) rc = dalloc(sizeof(long long), (void**)&b->step_dev);
+  for (int s = 0; s < 2 && !rc; ++s) {
+    bl.nA[s] = pA[s][njobs];
+    bl.nC[s] = pC[s][njobs];
+    rc = upload_pref(pA[s], &bl.prefA[s]);
+    if (!rc) rc = upload_pref(pC[s], &bl.prefC[s]);
+    for (int v = 0; v < BV_COUNT && !rc; ++v) {
+      bl.nB[s][v] = pB[s][v][njobs];
+      rc = upload_pref(pB[s][v], &bl.prefB[s][v]);
+    }
+  }
+  if (!rc) {
+    hipError_t e = hipMemcpy(b->jobs_dev, host.data(), host.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = hip_fail(e, "upload jobs");
+  }
+  if (!rc) {
+    hipError_t e = hipMemset(b->step_dev, 0, sizeof(long long));
+    if (e != hipSuccess) rc = hip_fail(e, "clear step counter");
+  }
+  if (rc) {
+    for (void* p : b->allocs) (void)hipFree(p);
+    delete b;
+    return rc;
+  }
+  *out = reinterpret_cast<mlv_batch*>(b);
+  return MLV_OK;
+}
+
+int mlv_batch_destroy(mlv_batch* bp) {
+  if (!bp) return MLV_OK;
+  BatchHost* b = reinterpret_cast<BatchHost*>(bp);
+  (void)hipDeviceSynchronize();
+  for (void* p : b->allocs) (void)hipFree(p);
+  delete b;
+  return MLV_OK;
+}
+
+int mlv_batch_step(mlv_batch* bp, int nsteps, int write_phys, void* stream) {
+  if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
+  if (nsteps < 0) return fail(MLV_ERR_ARG, "nsteps < 0");
+  BatchHost* b = reinterpret_cast<BatchHost*>(bp);
+  hipStream_t s = (hipStream_t)stream;
+  for (int k = 0; k < nsteps; ++k) {
+    const int wp = (write_phys && k == nsteps - 1) ? 1 : 0;
+    for (int st = 0; st < b->nstage; ++st) {
+      hipError_t e =
+          b->dtype == MLV_F64
+              ? fused_stage<double>((const JobDev<double>*)b->jobs_dev, b->bl, st, st == 0 ? wp : 0,
+                                    b->step_dev, (const cx<double>*)b->tw, s)
+              : fused_stage<float>((const JobDev<float>*)b->jobs_dev, b->bl, st, st == 0 ? wp : 0,
+                                   b->step_dev, (const cx<float>*)b->tw, s);
+      if (e != hipSuccess) return hip_fail(e, "fused stage");
+    }
+    hipError_t e = b->dtype == MLV_F64
+                       ? fused_advance<double>(nullptr, b->njobs, b->step_dev, nullptr, s)
+                       : fused_advance<float>(nullptr, b->njobs, b->step_dev, nullptr, s);
+    if (e != hipSuccess) return hip_fail(e, "advance");
+    b->steps += 1;
+  }
+  return MLV_OK;
+}
+
+int mlv_batch_set_dt(mlv_batch* bp, int job, double dt, void* stream) {
+  if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
+  BatchHost* b = reinterpret_cast<BatchHost*>(bp);
+  if (job < 0 || job >= b->njobs) return fail(MLV_ERR_ARG, "job index out of range");
+  if (!(dt > 0)) return fail(MLV_ERR_ARG, "dt must be positive");
+  hipError_t e = b->dtype == MLV_F64
+                     ? fused_set_dt<double>((JobDev<double>*)b->jobs_dev, job, dt, (hipStream_t)stream)
+                     : fused_set_dt<float>((JobDev<float>*)b->jobs_dev, job, dt, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "set dt");
+  return MLV_OK;
+}
+
+long long mlv_batch_steps(const mlv_batch* bp) {
+  if (!bp) return -1;
+  return reinterpret_cast<const BatchHost*>(bp)->steps;
+}
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+// Workgroup-level batched power-of-two FFT held in LDS.
+//
+// Layout: `slots` transforms of length N = 2^LOGN, slot f starts at
+// lds + f * ldsn(N); element i of a slot lives at pad(i) so the strided
+// Stockham writes do not pile onto one LDS bank.
+//
+// Algorithm: Stockham autosort, radix-16 passes followed by one radix
+// 2/4/8 pass for the remainder.  Every thread owns P = 16 or 32 points per
+// pass (P/R butterflies of radix R), so a block of NT threads transforms
+// NT*P/N slots at once.  A pass reads its R inputs into registers,
+// barriers, and writes the R outputs back in place, so one LDS buffer
+// suffices.  Twiddles come from an FP64-tabulated table (rounded once to
+// float for single precision); the in-register radix-R kernels use exact
+// constants for the trivial angles.
+#pragma once
+#include "common.hpp"
+
+namespace mlv {
+
+__host__ __device__ constexpr int fft_pad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int fft_ldsn(int n) { return n + (n >> 4); }
+
+// cos(2*pi*j/16), j = 0..15.
+__device__ constexpr double kCos16[16] = {
+    1.0,
+    0.92387953251128675613,
+    0.70710678118654752440,
+    0.38268343236508977173,
+    0.0,
+    -0.38268343236508977173,
+    -0.70710678118654752440,
+    -0.92387953251128675613,
+    -1.0,
+    -0.92387953251128675613,
+    -0.70710678118654752440,
+    -0.38268343236508977173,
+    0.0,
+    0.38268343236508977173,
+    0.70710678118654752440,
+    0.92387953251128675613};
+
+// x * exp(DIR*2*pi*i*k/R) for compile-time R <= 16 (k known after unrolling).
+template <int R, int DIR, typename T>
+__device__ __forceinline__ cx<T> rot(int k, cx<T> x) {
+  if (k == 0) return x;
+  if (4 * k == R) return DIR > 0 ? mk<T>(-x.y, x.x) : mk<T>(x.y, -x.x);
+  const int j = k * (16 / R);
+  const T c = T(kCos16[j]);
+  const T s = T(DIR) * T(kCos16[(j + 12) & 15]);
+  return mk<T>(x.x * c - x.y * s, x.x * s + x.y * c);
+}
+
+// In-register radix-R DFT, R a power of two <= 16 (recursive radix-2 DIT).
+template <int R, int DIR, typename T>
+struct Dft {
+  __device__ __forceinline__ static void run(cx<T>* v) {
+    cx<T> e[R / 2], o[R / 2];
+#pragma unroll
+    for (int i = 0; i < R / 2; ++i) {
+      e[i] = v[2 * i];
+      o[i] = v[2 * i + 1];
+    }
+    Dft<R / 2, DIR, T>::run(e);
+    Dft<R / 2, DIR, T>::run(o);
+#pragma unroll
+    for (int k = 0; k < R / 2; ++k) {
+      const cx<T> t = rot<R, DIR, T>(k, o[k]);
+      v[k] = cadd<T>(e[k], t);
+      v[k + R / 2] = csub<T>(e[k], t);
+    }
+  }
+};
+template <int DIR, typename T>
+struct Dft<1, DIR, T> {
+  __device__ __forceinline__ static void run(cx<T>*) {}
+};
+
+// One Stockham pass: radix 2^LOGR, sub-transform length Ns = 2^LOGNS.
+template <typename T, int LOGN, int LOGR, int LOGNS, int DIR, int NT, int P>
+__device__ __forceinline__ void fft_pass(cx<T>* __restrict__ lds, int active,
+                                         const cx<T>* __restrict__ tw) {
+  constexpr int N = 1 << LOGN;
+  constexpr int R = 1 << LOGR;
+  constexpr int NS = 1 << LOGNS;
+  constexpr int BF = N / R;          // butterflies per transform
+  constexpr int ITEMS = P / R;       // butterflies per thread
+  constexpr int LDSN = fft_ldsn(N);
+  const int tid = threadIdx.x;
+  cx<T> v[ITEMS][R];
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const int w = tid + it * NT;
+    const int f = w / BF;
+    const int j = w % BF;
+    if (f < active) {
+      const cx<T>* src = lds + f * LDSN;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[it][r] = src[fft_pad(j + r * BF)];
+      if (NS > 1) {
+        const int jm = j & (NS - 1);
+#pragma unroll
+        for (int r = 1; r < R; ++r)
+          v[it][r] = cmul<T>(v[it][r], twiddle<T, DIR>(tw, jm * r, LOGNS + LOGR));
+      }
+      Dft<R, DIR, T>::run(v[it]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < ITEMS; ++it) {
+    const int w = tid + it * NT;
+    const int f = w / BF;
+    const int j = w % BF;
+    if (f < active) {
+      cx<T>* dst = lds + f * LDSN;
+      const int jm = j & (NS - 1);
+      const int base = (j - jm) * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[fft_pad(base + r * NS)] = v[it][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int LOGN, int LOGNS, int DIR, int NT, int P>
+struct FftPasses {
+  static constexpr int REM = LOGN - LOGNS;
+  static constexpr int LOGR = REM >= 4 ? 4 : REM;
+  __device__ __forceinline__ static void run(cx<T>* lds, int active, const cx<T>* tw) {
+    if constexpr (REM > 0) {
+      fft_pass<T, LOGN, LOGR, LOGNS, DIR, NT, P>(lds, active, tw);
+      FftPasses<T, LOGN, LOGNS + LOGR, DIR, NT, P>::run(lds, active, tw);
+    }
+  }
+};
+
+// Unnormalised DFT of `active` slots in place.  DIR = -1 forward
+// (exp(-i...)), +1 inverse.  Requires NT*P/N >= active and a barrier
+// between the writes that filled the slots and this call.
+template <typename T, int LOGN, int DIR, int NT, int P>
+__device__ __forceinline__ void fft_lds(cx<T>* lds, int active, const cx<T>* tw) {
+  static_assert(P == 16 || P == 32, "points per thread");
+  static_assert((NT * P) % (1 << LOGN) == 0 || (1 << LOGN) > NT * P, "slots");
+  FftPasses<T, LOGN, 0, DIR, NT, P>::run(lds, active, tw);
+}
+
+// Slots available to a block of NT threads at P points per thread.
+template <int LOGN, int NT, int P>
+__host__ __device__ constexpr int fft_slots() {
+  return (NT * P) >> LOGN;
+}
+
+// Points per thread used for a transform length: 16 up to 1024, 32 beyond.
+__host__ __device__ constexpr int fft_points(int logn) { return logn >= 11 ? 32 : 16; }
+
+}  // namespace mlv

@@ -1,0 +1,430 @@
+// Fused fully-spectral step kernels over a job table (see fused.hpp).
+#include "fused.hpp"
+
+namespace mlv {
+
+__device__ __forceinline__ int find_job(const int* __restrict__ pref, int njobs, int b) {
+  int lo = 0, hi = njobs;  // pref[lo] <= b < pref[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pref[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int ring(long long c0, long long step, int q) {
+  return (int)((c0 + step) % q);
+}
+
+// --------------------------------------------------------------------------
+// Kernel A: spectral prologue + inverse x pass
+// --------------------------------------------------------------------------
+template <typename T, int LOGN>
+__device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S, int local,
+                                           int par, const cx<T>* __restrict__ tw, cx<T>* lds) {
+  constexpr int N = 1 << LOGN;
+  constexpr int P = fft_points(LOGN);
+  constexpr int TC = fft_slots<LOGN, kNT, P>();
+  constexpr int LDSN = fft_ldsn(N);
+  const int nn = J.nn, nm = J.nm, rows = J.rows;
+  const int ntile = (nm + TC - 1) / TC;
+  const int opi = local / ntile, tile = local % ntile;
+  const XinvOp op = S.a[opi];
+  const int c0 = tile * TC;
+  const int ncol = min(TC, nm - c0);
+  const cx<T>* __restrict__ src = J.state[op.from_new ? (par ^ 1) : par][op.src];
+  cx<T>* __restrict__ dst = J.work + (size_t)op.ch * N * nm;
+  cx<T>* __restrict__ psi_out = (op.op == XOP_UX && op.src == 0 && !op.from_new) ? J.psi : nullptr;
+  const double fx = J.fx, fz = J.fz, fx2 = J.fx2, fz2 = J.fz2;
+  for (int e = threadIdx.x; e < N * TC; e += kNT) {
+    const int c = e % TC, x = e / TC;
+    cx<T> v = czero<T>();
+    if (c < ncol) {
+      const int r = x <= nn ? x : (x >= N - nn ? x - N + rows : -1);
+      if (r >= 0) {
+        const int m = c0 + c;
+        const size_t idx = (size_t)r * nm + m;
+        const cx<T> f = src[idx];
+        if (op.op == XOP_ID) {
+          v = f;
+        } else {
+          const int n = r <= nn ? r : r - rows;
+          double lap = fx2 * (double)(n * n) + fz2 * (double)(m * m);
+          if (r == 0 && m == 0) lap = 1.0;
+          const T rl = (T)(1.0 / lap);
+          // psi = solve(-f) (utility.py:65, LaplacianSolver.py:64-66)
+          const cx<T> ps = mk<T>(-f.x * rl, -f.y * rl);
+          if (psi_out) psi_out[idx] = ps;
+          if (op.op == XOP_UX) {  // ux = -d(psi)/dz
+            const T y = (T)(fz * m);
+            v = mk<T>(y * ps.y, -(y * ps.x));
+          } else {  // uz = d(psi)/dx
+            const T y = (T)(fx * n);
+            v = mk<T>(-(y * ps.y), y * ps.x);
+          }
+        }
+      }
+    }
+    lds[c * LDSN + fft_pad(x)] = v;
+  }
+  __syncthreads();
+  fft_lds<T, LOGN, +1, kNT, P>(lds, ncol, tw);
+  for (int e = threadIdx.x; e < N * TC; e += kNT) {
+    const int c = e % TC, x = e / TC;
+    if (c < ncol) dst[(size_t)x * nm + c0 + c] = lds[c * LDSN + fft_pad(x)];
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kNT) k_stepA(const JobDev<T>* __restrict__ jobs,
+                                               const int* __restrict__ pref, int njobs, int stage,
+                                               const long long* __restrict__ step,
+                                               const cx<T>* __restrict__ tw) {
+  extern __shared__ char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int j = find_job(pref, njobs, blockIdx.x);
+  const JobDev<T>& J = jobs[j];
+  const int local = blockIdx.x - pref[j];
+  const int par = (int)((J.par0 + *step) & 1);
+  switch (J.lognx) {
+#define X(L)                                                       \
+  case L:                                                          \
+    stepA_body<T, L>(J, J.st[stage], local, par, tw, lds);         \
+    break;
+    X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+#undef X
+    default:
+      break;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Kernel B: z pass, c2r of the channels, products, r2c of the products
+// --------------------------------------------------------------------------
+template <typename T, int LOGN, int V>
+__device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S, int local,
+                                           int wphys, const cx<T>* __restrict__ tw, cx<T>* lds) {
+  using Pg = BProg<V>;
+  constexpr int NIN = Pg::NIN, NEQ = Pg::NEQ;
+  constexpr int N = 1 << LOGN;
+  constexpr int P = fft_points(LOGN);
+  constexpr int SL = fft_slots<LOGN, kNT, P>();
+  constexpr int LDSN = fft_ldsn(N);
+  if constexpr (!bfits(NIN, NEQ, SL)) {
+    return;  // rejected on the host
+  } else {
+    constexpr int RB = brows(NIN, NEQ, SL);
+    const int nx = J.nx, nm = J.nm;
+    const size_t chs = (size_t)nx * nm;
+    const int x0 = local * RB;
+    const int nrows = min(RB, nx - x0);
+    const int nseq = nrows * NIN;
+    const int nslot = (nseq + 1) / 2;
+    cx<T>* __restrict__ work = J.work;
+    const int nzero = N - 2 * nm + 1;
+    for (int e = threadIdx.x; e < nslot * nzero; e += kNT) {
+      const int s = e / nzero, k = nm + e % nzero;
+      lds[s * LDSN + fft_pad(k)] = czero<T>();
+    }
+    for (int e = threadIdx.x; e < nslot * nm; e += kNT) {
+      const int s = e / nm, k = e % nm;
+      const int qa = 2 * s, qb = qa + 1;
+      const cx<T> a = work[(size_t)(qa % NIN) * chs + (size_t)(x0 + qa / NIN) * nm + k];
+      const cx<T> b = qb < nseq
+                          ? work[(size_t)(qb % NIN) * chs + (size_t)(x0 + qb / NIN) * nm + k]
+                          : czero<T>();
+      cx<T>* z = lds + s * LDSN;
+      if (k == 0) {
+        z[0] = mk<T>(a.x, b.x);
+      } else {
+        z[fft_pad(k)] = mk<T>(a.x - b.y, a.y + b.x);
+        z[fft_pad(N - k)] = mk<T>(a.x + b.y, b.x - a.y);
+      }
+    }
+    __syncthreads();
+    fft_lds<T, LOGN, +1, kNT, P>(lds, nslot, tw);
+    for (int n = threadIdx.x; n < N; n += kNT) {
+      T v[RB * NIN];
+#pragma unroll
+      for (int q = 0; q < RB * NIN; ++q) {
+        if (q < nseq) {
+          const cx<T> z = lds[(q >> 1) * LDSN + fft_pad(n)];
+          v[q] = (q & 1) ? z.y : z.x;
+        } else {
+          v[q] = T(0);
+        }
+      }
+      if (wphys) {
+#pragma unroll
+        for (int q = 0; q < RB * NIN; ++q) {
+          if (q < nseq) {
+#pragma unroll
+            for (int i = 0; i < kMaxPhys; ++i)
+              if (S.phys_ch[i] == q % NIN) J.phys[i][(size_t)(x0 + q / NIN) * N + n] = v[q];
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        if (r < nrows) {
+          T px[NEQ], pz[NEQ];
+          Pg::template products<T>(&v[r * NIN], px, pz);
+#pragma unroll
+          for (int e = 0; e < NEQ; ++e) lds[(r * NEQ + e) * LDSN + fft_pad(n)] = mk<T>(px[e], pz[e]);
+        }
+      }
+    }
+    __syncthreads();
+    fft_lds<T, LOGN, -1, kNT, P>(lds, nrows * NEQ, tw);
+    for (int e = threadIdx.x; e < nrows * NEQ * nm; e += kNT) {
+      const int o = e / nm, k = e % nm;
+      const int r = o / NEQ, eq = o % NEQ;
+      const cx<T>* z = lds + o * LDSN;
+      const cx<T> y = z[fft_pad(k)];
+      const cx<T> ym = z[fft_pad((N - k) & (N - 1))];
+      const size_t off = (size_t)(x0 + r) * nm + k;
+      work[(size_t)pchan(NIN, eq, 0) * chs + off] = mk<T>(T(0.5) * (y.x + ym.x), T(0.5) * (y.y - ym.y));
+      work[(size_t)pchan(NIN, eq, 1) * chs + off] = mk<T>(T(0.5) * (y.y + ym.y), T(0.5) * (ym.x - y.x));
+    }
+  }
+}
+
+template <typename T, int V>
+__global__ void __launch_bounds__(kNT) k_stepB(const JobDev<T>* __restrict__ jobs,
+                                               const int* __restrict__ pref, int njobs, int stage,
+                                               int wphys, const cx<T>* __restrict__ tw) {
+  extern __shared__ char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int j = find_job(pref, njobs, blockIdx.x);
+  const JobDev<T>& J = jobs[j];
+  const int local = blockIdx.x - pref[j];
+  switch (J.lognz) {
+#define X(L)                                                      \
+  case L:                                                         \
+    stepB_body<T, L, V>(J, J.st[stage], local, wphys, tw, lds);   \
+    break;
+    X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+#undef X
+    default:
+      break;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Kernel C: forward x pass, FD operator, sources, AB + semi-implicit update
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T modk(const cx<T>* __restrict__ tw, int k, int logn, int order,
+                                  double h) {
+  const T s1 = tsin<T>(tw, k, logn);
+  if (order == 2) return s1 / (T)h;
+  const T s2 = tsin<T>(tw, 2 * k, logn);
+  return (T(8) * s1 - s2) / (T)(6.0 * h);
+}
+
+template <typename T, int LOGN>
+__device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S, int local, int cur,
+                                           int par, const cx<T>* __restrict__ tw, cx<T>* lds) {
+  constexpr int N = 1 << LOGN;
+  constexpr int P = fft_points(LOGN);
+  constexpr int SL = fft_slots<LOGN, kNT, P>();
+  constexpr int TC = SL / 2;
+  constexpr int LDSN = fft_ldsn(N);
+  const int nn = J.nn, nm = J.nm, rows = J.rows;
+  const size_t chs = (size_t)N * nm;
+  const size_t ssz = (size_t)rows * nm;
+  const int ntile = (nm + TC - 1) / TC;
+  const int eqi = local / ntile, tile = local % ntile;
+  const int field = S.eq_field[eqi];
+  const int c0 = tile * TC;
+  const int ncol = min(TC, nm - c0);
+  const cx<T>* __restrict__ pxc = J.work + (size_t)pchan(J.pbase, eqi, 0) * chs;
+  const cx<T>* __restrict__ pzc = J.work + (size_t)pchan(J.pbase, eqi, 1) * chs;
+  for (int e = threadIdx.x; e < N * TC; e += kNT) {
+    const int c = e % TC, x = e / TC;
+    cx<T> a = czero<T>(), b = czero<T>();
+    if (c < ncol) {
+      a = pxc[(size_t)x * nm + c0 + c];
+      b = pzc[(size_t)x * nm + c0 + c];
+    }
+    lds[(2 * c) * LDSN + fft_pad(x)] = a;
+    lds[(2 * c + 1) * LDSN + fft_pad(x)] = b;
+  }
+  __syncthreads();
+  fft_lds<T, LOGN, -1, kNT, P>(lds, 2 * ncol, tw);
+
+  const int q = J.q;
+  const double dt = J.dt, alpha = J.alpha;
+  const double c1 = (1 - alpha) * dt, c2 = alpha * dt;
+  const double hab = q == 2 ? dt / 2 : dt / 24;
+  const double coeff = J.coeff[field];
+  const T scale = (T)J.scale;
+  const cx<T>* __restrict__ sin_ = J.state[par][field];
+  cx<T>* __restrict__ sout = J.state[par ^ 1][field];
+  cx<T>* __restrict__ hist = J.hist[field];
+  for (int e = threadIdx.x; e < rows * TC; e += kNT) {
+    const int c = e % TC, r = e / TC;
+    if (c >= ncol) continue;
+    const int x = r <= nn ? r : r + N - rows;
+    const int n = r <= nn ? r : r - rows;
+    const int m = c0 + c;
+    const size_t idx = (size_t)r * nm + m;
+    const cx<T> X = lds[(2 * c) * LDSN + fft_pad(x)];
+    const cx<T> Z = lds[(2 * c + 1) * LDSN + fft_pad(x)];
+    const T sx = modk<T>(tw, n, LOGN, J.fd_order, J.dx);
+    const T sz = modk<T>(tw, m, J.lognz, J.fd_order, J.dz);
+    // i * (sx X + sz Z) / (nx nz)
+    cx<T> rhs = mk<T>(-(sx * X.y + sz * Z.y) * scale, (sx * X.x + sz * Z.x) * scale);
+    if (J.problem == MLV_PROB_DDC) {
+      const double Pr = J.param[0], R0 = J.param[1];
+      if (field == 0) {
+        const T y = (T)(J.fx * n);
+        const cx<T> s = J.state[par][2][idx], t = J.state[par][1][idx];
+        const T pr = (T)Pr;
+        rhs = cadd<T>(rhs, mk<T>(pr * -(y * s.y), pr * (y * s.x)));
+        rhs = csub<T>(rhs, mk<T>(pr * -(y * t.y), pr * (y * t.x)));
+      } else {
+        double lap = J.fx2 * (double)(n * n) + J.fz2 * (double)(m * m);
+        if (r == 0 && m == 0) lap = 1.0;
+        const T rl = (T)(1.0 / lap);
+        const cx<T> w = J.state[par][0][idx];
+        const cx<T> ps = mk<T>(-w.x * rl, -w.y * rl);
+        const T y = (T)(J.fx * n);
+        cx<T> uz = mk<T>(-(y * ps.y), y * ps.x);
+        if (field == 2) {
+          const T ir0 = (T)(1.0 / R0);
+          uz = mk<T>(uz.x * ir0, uz.y * ir0);
+        }
+        rhs = csub<T>(rhs, uz);
+      }
+    }
+    hist[(size_t)cur * ssz + idx] = rhs;
+    auto hget = [&](int i) {
+      int s = cur + i;
+      if (s < 0) s += q;
+      return hist[(size_t)s * ssz + idx];
+    };
+    T abx, aby;
+    {
+      const cx<T> d1 = hget(-1);
+      if (q == 2) {
+        abx = T(3) * rhs.x - d1.x;
+        aby = T(3) * rhs.y - d1.y;
+      } else {
+        const cx<T> d2 = hget(-2), d3 = hget(-3);
+        abx = T(55) * rhs.x - T(59) * d1.x + T(37) * d2.x - T(9) * d3.x;
+        aby = T(55) * rhs.y - T(59) * d1.y + T(37) * d2.y - T(9) * d3.y;
+      }
+    }
+    const double lin = coeff * (J.fx2 * (double)(n * n) + J.fz2 * (double)(m * m));
+    const T num = (T)(1.0 + c1 * lin);
+    const T rden = (T)(1.0 / (1.0 - c2 * lin));
+    const cx<T> v = sin_[idx];
+    cx<T> nv = mk<T>((num * v.x + (T)hab * abx) * rden, (num * v.y + (T)hab * aby) * rden);
+    if (J.problem == MLV_PROB_DDC && field != 0 && m == 0) nv = czero<T>();
+    sout[idx] = nv;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kNT) k_stepC(const JobDev<T>* __restrict__ jobs,
+                                               const int* __restrict__ pref, int njobs, int stage,
+                                               const long long* __restrict__ step,
+                                               const cx<T>* __restrict__ tw) {
+  extern __shared__ char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int j = find_job(pref, njobs, blockIdx.x);
+  const JobDev<T>& J = jobs[j];
+  const int local = blockIdx.x - pref[j];
+  const long long st = *step;
+  const int par = (int)((J.par0 + st) & 1);
+  const int cur = ring(J.curr0, st, J.q);
+  switch (J.lognx) {
+#define X(L)                                                            \
+  case L:                                                               \
+    stepC_body<T, L>(J, J.st[stage], local, cur, par, tw, lds);         \
+    break;
+    X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+#undef X
+    default:
+      break;
+  }
+}
+
+__global__ void k_advance(long long* step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+template <typename T>
+__global__ void k_set_dt(JobDev<T>* jobs, int j, double dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) jobs[j].dt = dt;
+}
+
+// --------------------------------------------------------------------------
+// Host side
+// --------------------------------------------------------------------------
+#define MLV_SET_LDS_ONCE(kern, bytes)                                                     \
+  do {                                                                                    \
+    static size_t done_ = 0;                                                              \
+    if (done_ < (bytes)) {                                                                \
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),                      \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)(bytes)); \
+      done_ = (bytes);                                                                    \
+    }                                                                                     \
+  } while (0)
+
+template <typename T, int V>
+static void launchB(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int wphys,
+                    const cx<T>* tw, hipStream_t s) {
+  if (bl.nB[stage][V] == 0) return;
+  MLV_SET_LDS_ONCE((k_stepB<T, V>), bl.ldsB[V]);
+  hipLaunchKernelGGL((k_stepB<T, V>), dim3(bl.nB[stage][V]), dim3(kNT), bl.ldsB[V], s, jobs,
+                     (const int*)bl.prefB[stage][V], bl.njobs, stage, wphys, tw);
+}
+
+template <typename T>
+hipError_t fused_stage(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int wphys,
+                       const long long* step, const cx<T>* tw, hipStream_t s) {
+  if (bl.nA[stage] > 0) {
+    MLV_SET_LDS_ONCE(k_stepA<T>, bl.ldsA);
+    hipLaunchKernelGGL(k_stepA<T>, dim3(bl.nA[stage]), dim3(kNT), bl.ldsA, s, jobs,
+                       (const int*)bl.prefA[stage], bl.njobs, stage, step, tw);
+  }
+  launchB<T, BV_VORT1>(jobs, bl, stage, wphys, tw, s);
+  launchB<T, BV_VORT2>(jobs, bl, stage, wphys, tw, s);
+  launchB<T, BV_VORT3>(jobs, bl, stage, wphys, tw, s);
+  launchB<T, BV_MHD1>(jobs, bl, stage, wphys, tw, s);
+  launchB<T, BV_MHD2>(jobs, bl, stage, wphys, tw, s);
+  if (bl.nC[stage] > 0) {
+    MLV_SET_LDS_ONCE(k_stepC<T>, bl.ldsC);
+    hipLaunchKernelGGL(k_stepC<T>, dim3(bl.nC[stage]), dim3(kNT), bl.ldsC, s, jobs,
+                       (const int*)bl.prefC[stage], bl.njobs, stage, step, tw);
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t fused_advance(JobDev<T>*, int, long long* step, double*, hipStream_t s) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, step);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t fused_set_dt(JobDev<T>* jobs, int job, double dt, hipStream_t s) {
+  hipLaunchKernelGGL(k_set_dt<T>, dim3(1), dim3(64), 0, s, jobs, job, dt);
+  return hipGetLastError();
+}
+
+template hipError_t fused_set_dt<float>(JobDev<float>*, int, double, hipStream_t);
+template hipError_t fused_set_dt<double>(JobDev<double>*, int, double, hipStream_t);
+template hipError_t fused_stage<float>(const JobDev<float>*, const BatchLaunch&, int, int,
+                                       const long long*, const cx<float>*, hipStream_t);
+template hipError_t fused_stage<double>(const JobDev<double>*, const BatchLaunch&, int, int,
+                                        const long long*, const cx<double>*, hipStream_t);
+template hipError_t fused_advance<float>(JobDev<float>*, int, long long*, double*, hipStream_t);
+template hipError_t fused_advance<double>(JobDev<double>*, int, long long*, double*, hipStream_t);
+
+}  // namespace mlv

@@ -1,0 +1,154 @@
+// Fused fully-spectral timestep over a table of jobs (simulations).
+//
+// One stage of a step is three kernels:
+//   A  x pass inverse: per output channel, a spectral prologue on a state
+//      field (identity, or velocity from the streamfunction psi = -f/lap),
+//      zero-pad |kx| <= nn to nx, inverse FFT along x -> channel (nx, nm).
+//   B  z pass per row block: c2r of every channel row, physical products
+//      (the flux terms sign * u * f of each equation), r2c of the products,
+//      keep kz < nm -> Px_e, Pz_e channels (nx, nm).  Optionally writes the
+//      physical rows of the input channels (for CFL and diagnostics).
+//   C  x pass forward per equation: FFT along x of Px_e and Pz_e, keep
+//      |kx| <= nn, apply the periodic finite-difference operator as its
+//      exact modified wavenumber, add the equation's spectral source terms,
+//      store the RHS into the Adams-Bashforth ring, and apply the
+//      semi-implicit update, writing the new state to the other buffer of
+//      a ping-pong pair.
+//
+// The periodic central difference of a periodic grid function is a
+// circulant operator, so on the DFT grid it is exactly multiplication by
+// i*sin(k*h)/h (order 2) or i*(8 sin(k h) - sin(2 k h))/(6 h) (order 4).
+// Applying it after the forward transform is therefore the same discrete
+// operator as the reference's physical-space stencil followed by rfft2
+// (Variable.py:125-127, SpatialDifferentiator.py:76-104,130-185), without
+// a halo exchange between rows.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "../../include/melvin_hip.h"
+#include "transforms.hpp"
+
+namespace mlv {
+
+constexpr int kMaxFields = 4;
+constexpr int kMaxChan = 8;   // input channels of the z pass
+constexpr int kMaxEq = 3;
+constexpr int kMaxA = 8;
+constexpr int kMaxPhys = 8;
+
+enum XinvOpKind : int { XOP_ID = 0, XOP_UX = 1, XOP_UZ = 2 };
+
+struct XinvOp {
+  int src;       // field index
+  int op;        // XinvOpKind
+  int ch;        // destination channel
+  int from_new;  // read the field from the freshly written state buffer
+};
+
+// z-pass variants (compile-time product programs)
+enum BVariant : int { BV_VORT1 = 0, BV_VORT2 = 1, BV_VORT3 = 2, BV_MHD1 = 3, BV_MHD2 = 4, BV_COUNT = 5 };
+
+struct StageDev {
+  int nA;
+  XinvOp a[kMaxA];
+  int bvar;      // BVariant
+  int neq;       // equations updated by kernel C
+  int eq_field[kMaxEq];
+  int phys_ch[kMaxPhys];  // z-pass input index written to phys[i] (-1 = none)
+};
+
+template <typename T>
+struct JobDev {
+  int problem, nstage;
+  int nx, nz, lognx, lognz, nn, nm, rows;
+  int q, curr0, par0, fd_order;
+  int pbase;                 // first workspace channel of the Px/Pz outputs
+  double fx, fz, fx2, fz2;   // 2*pi/L and -(2*pi)^2/L^2, as the reference forms them
+  double dx, dz, scale;      // grid spacings, 1/(nx*nz)
+  double alpha, dt;
+  double coeff[kMaxFields];  // diffusion coefficient of each field's equation
+  double param[4];           // problem parameters (DDC: Pr, R0)
+  int nfield;
+  cx<T>* state[2][kMaxFields];
+  cx<T>* hist[kMaxFields];
+  cx<T>* work;
+  cx<T>* psi;
+  T* phys[kMaxPhys];
+  StageDev st[2];
+};
+
+// Per-variant channel/product description used by kernel B.
+template <int V> struct BProg;
+template <int NF> struct BVortProg {
+  static constexpr int NIN = NF + 2;  // fields, ux, uz
+  static constexpr int NEQ = NF;
+  template <typename T>
+  __device__ __forceinline__ static void products(const T* v, T* px, T* pz) {
+#pragma unroll
+    for (int e = 0; e < NF; ++e) {
+      px[e] = -(v[NF] * v[e]);
+      pz[e] = -(v[NF + 1] * v[e]);
+    }
+  }
+};
+template <> struct BProg<BV_VORT1> : BVortProg<1> {};
+template <> struct BProg<BV_VORT2> : BVortProg<2> {};
+template <> struct BProg<BV_VORT3> : BVortProg<3> {};
+// MHD channels: 0 w, 1 j, 2 ux, 3 uz, 4 bx, 5 bz
+template <> struct BProg<BV_MHD1> {
+  static constexpr int NIN = 6;
+  static constexpr int NEQ = 1;
+  template <typename T>
+  __device__ __forceinline__ static void products(const T* v, T* px, T* pz) {
+    px[0] = -(v[2] * v[0]) + v[4] * v[1];
+    pz[0] = -(v[3] * v[0]) + v[5] * v[1];
+  }
+};
+template <> struct BProg<BV_MHD2> {
+  static constexpr int NIN = 6;
+  static constexpr int NEQ = 1;
+  template <typename T>
+  __device__ __forceinline__ static void products(const T* v, T* px, T* pz) {
+    px[0] = -(v[2] * v[1]) + v[4] * v[0];
+    pz[0] = -(v[3] * v[1]) + v[5] * v[0];
+  }
+};
+
+// Rows per z-pass block: as many as fit the LDS slots, at most 8.
+__host__ __device__ constexpr int brows(int nin, int neq, int slots) {
+  int best = 1;
+  for (int r = 1; r <= 8; ++r) {
+    const int need_in = (r * nin + 1) / 2;
+    const int need_out = r * neq;
+    const int need = need_in > need_out ? need_in : need_out;
+    if (need <= slots) best = r;
+  }
+  return best;
+}
+__host__ __device__ constexpr bool bfits(int nin, int neq, int slots) {
+  return ((nin + 1) / 2 <= slots) && (neq <= slots);
+}
+__host__ __device__ constexpr int bvar_nin(int v) { return v <= BV_VORT3 ? v + 3 : 6; }
+__host__ __device__ constexpr int bvar_neq(int v) { return v <= BV_VORT3 ? v + 1 : 1; }
+
+// workspace channel of equation e's Px (k=0) / Pz (k=1) output; the
+// z-pass inputs occupy channels [0, base)
+__host__ __device__ constexpr int pchan(int base, int e, int k) { return base + 2 * e + k; }
+
+struct BatchLaunch {
+  int njobs;
+  int nA[2], nB[2][BV_COUNT], nC[2];  // total blocks per launch
+  size_t ldsA, ldsB[BV_COUNT], ldsC;
+  int* prefA[2];                       // device prefix arrays (njobs + 1)
+  int* prefB[2][BV_COUNT];
+  int* prefC[2];
+};
+
+template <typename T>
+hipError_t fused_stage(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int wphys,
+                       const long long* step, const cx<T>* tw, hipStream_t s);
+template <typename T>
+hipError_t fused_advance(JobDev<T>* jobs, int njobs, long long* step, double* t, hipStream_t s);
+template <typename T>
+hipError_t fused_set_dt(JobDev<T>* jobs, int job, double dt, hipStream_t s);
+
+}  // namespace mlv

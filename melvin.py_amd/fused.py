@@ -1,0 +1,245 @@
+"""Fused whole-step kernels over batches of simulations.
+
+A ``Job`` is one fully spectral simulation (its state, Adams-Bashforth
+ring and workspace live in GPU memory); a ``Batch`` advances any set of
+jobs -- of mixed grid sizes and problems, one precision -- with three HIP
+kernels per stage and step (two stages for MHD).  One step of a job is the
+whole loop body of the matching reference example:
+
+* ``"vorticity"``: omega plus up to two passive scalars sharing the
+  velocity (taylor_green_vortex.py:87-93, vortex_pairs.py:122-128,
+  kelvin_helmholtz_instability.py:117-129 with the ink block enabled);
+* ``"ddc"``: omega, T, S of double_diffusive_convection.py:102-124;
+* ``"mhd"``: omega and j of resistive_tearing_instability.py:127-146.
+
+Every step reads dt from the device job table, so steps can be queued
+back to back (and captured) without host round trips.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+
+PROBLEMS = {
+    "vorticity": _lib.PROB_VORTICITY,
+    "ddc": _lib.PROB_DDC,
+    "mhd": _lib.PROB_MHD,
+}
+FIELDS = {"ddc": 3, "mhd": 2}
+
+
+def _phys_count(problem, nfield):
+    return {"vorticity": nfield + 2, "ddc": 5, "mhd": 6}[problem]
+
+
+class Job:
+    """One simulation's device state.
+
+    ``coeff[i]`` is the diffusion coefficient of field i's equation (the
+    ``lin_op = coeff * lap()`` factor of the examples); ``param`` holds the
+    problem constants (DDC: Pr, R0).
+    """
+
+    def __init__(self, problem, nx, nz, lx, lz, dtype="f32", dt=1e-3, coeff=(0.0,),
+                 param=(), nfield=None, ab_order=2, fd_order=2, alpha=0.51,
+                 want_psi=False, want_phys=False, device=None):
+        if problem not in PROBLEMS:
+            raise ValueError(f"unknown problem {problem!r}")
+        self.problem = problem
+        self.nfield = FIELDS.get(problem, nfield if nfield is not None else len(coeff))
+        if problem == "vorticity" and not 1 <= self.nfield <= 3:
+            raise ValueError("vorticity jobs carry 1..3 fields")
+        self.nx, self.nz, self.lx, self.lz = int(nx), int(nz), float(lx), float(lz)
+        self.dtype = dtype
+        self.double = dtype == "f64"
+        if dtype not in ("f32", "f64"):
+            raise ValueError("dtype must be 'f32' or 'f64'")
+        self.real = torch.float64 if self.double else torch.float32
+        self.cplx = torch.complex128 if self.double else torch.complex64
+        self.dt = float(dt)
+        self.coeff = list(coeff) + [0.0] * (4 - len(coeff))
+        self.param = list(param) + [0.0] * (4 - len(param))
+        self.ab_order = int(ab_order)
+        self.fd_order = int(fd_order)
+        self.alpha = float(alpha)
+        self.nn = (self.nx - 1) // 3
+        self.nm = (self.nz - 1) // 3
+        self.rows = 2 * self.nn + 1
+        dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
+        shape = (self.rows, self.nm)
+        nf = self.nfield
+        self.buf = [[torch.zeros(shape, dtype=self.cplx, device=dev) for _ in range(nf)]
+                    for _ in range(2)]
+        self.hist = [torch.zeros((self.ab_order,) + shape, dtype=self.cplx, device=dev)
+                     for _ in range(nf)]
+        nbytes = ctypes.c_size_t()
+        _lib.call("mlv_job_work_bytes", PROBLEMS[problem], nf, self.nx, self.nz,
+                  _lib.F64 if self.double else _lib.F32, ctypes.byref(nbytes))
+        self.work = torch.empty(nbytes.value, dtype=torch.uint8, device=dev)
+        self.psi = torch.zeros(shape, dtype=self.cplx, device=dev) if want_psi else None
+        self.phys = ([torch.zeros((self.nx, self.nz), dtype=self.real, device=dev)
+                      for _ in range(_phys_count(problem, nf))] if want_phys else None)
+        self.curr = 0
+        self.parity = 0
+        self.steps = 0
+        self.t = 0.0
+
+    # -- host access --------------------------------------------------------
+    @property
+    def spectral_shape(self):
+        return (self.rows, self.nm)
+
+    def hbm_bytes(self):
+        n = sum(b.numel() * b.element_size() for row in self.buf for b in row)
+        n += sum(h.numel() * h.element_size() for h in self.hist)
+        n += self.work.numel()
+        if self.psi is not None:
+            n += self.psi.numel() * self.psi.element_size()
+        if self.phys is not None:
+            n += sum(p.numel() * p.element_size() for p in self.phys)
+        return n
+
+    def set_state(self, fields):
+        """Load spectral fields (host or device arrays) into the current buffer."""
+        if len(fields) != self.nfield:
+            raise ValueError(f"expected {self.nfield} fields")
+        for i, f in enumerate(fields):
+            t = torch.as_tensor(np.asarray(f) if not isinstance(f, torch.Tensor) else f)
+            if tuple(t.shape) != self.spectral_shape:
+                raise ValueError(f"field {i} has shape {tuple(t.shape)}, expected "
+                                 f"{self.spectral_shape}")
+            self.buf[self.parity][i].copy_(t.to(self.buf[0][0].device, self.cplx))
+
+    def current(self):
+        """Device tensors of the current spectral state."""
+        return self.buf[self.parity]
+
+    def state(self):
+        """Current spectral state as complex128 host arrays."""
+        return [b.detach().cpu().numpy().astype(np.complex128) for b in self.current()]
+
+    def physical(self):
+        """Physical fields written by the last step run with write_phys."""
+        if self.phys is None:
+            raise RuntimeError("job was created without physical outputs")
+        return [p.detach().cpu().numpy().astype(np.float64) for p in self.phys]
+
+    def desc(self):
+        d = _lib.JobDesc()
+        d.problem = PROBLEMS[self.problem]
+        d.nx, d.nz, d.lx, d.lz = self.nx, self.nz, self.lx, self.lz
+        d.fd_order, d.ab_order, d.alpha, d.dt = self.fd_order, self.ab_order, self.alpha, self.dt
+        for i in range(4):
+            d.coeff[i] = self.coeff[i]
+            d.param[i] = self.param[i]
+        d.nfield = self.nfield
+        for b in range(2):
+            for f in range(self.nfield):
+                d.state[b][f] = self.buf[b][f].data_ptr()
+        for f in range(self.nfield):
+            d.hist[f] = self.hist[f].data_ptr()
+        d.work = self.work.data_ptr()
+        d.psi = self.psi.data_ptr() if self.psi is not None else None
+        if self.phys is not None:
+            for i, p in enumerate(self.phys):
+                d.phys[i] = p.data_ptr()
+        d.curr, d.parity = self.curr, self.parity
+        return d
+
+
+class Batch:
+    """A set of jobs advanced together; all jobs share one precision."""
+
+    def __init__(self, jobs):
+        if not jobs:
+            raise ValueError("empty batch")
+        dt = {j.dtype for j in jobs}
+        if len(dt) != 1:
+            raise ValueError("all jobs of a batch must share one precision")
+        self.jobs = list(jobs)
+        self.double = self.jobs[0].double
+        arr = (_lib.JobDesc * len(self.jobs))(*[j.desc() for j in self.jobs])
+        h = ctypes.c_void_p()
+        _lib.call("mlv_batch_create", ctypes.byref(h), _lib.F64 if self.double else _lib.F32,
+                  arr, len(self.jobs))
+        self.handle = h
+
+    def step(self, nsteps=1, write_phys=False, stream=None):
+        """Queue ``nsteps`` steps of every job on the stream (asynchronous)."""
+        if nsteps <= 0:
+            return
+        _lib.call("mlv_batch_step", self.handle, int(nsteps), 1 if write_phys else 0,
+                  _lib.stream_handle(stream))
+        for j in self.jobs:
+            j.steps += nsteps
+            j.curr = (j.curr + nsteps) % j.ab_order
+            j.parity = (j.parity + nsteps) & 1
+            j.t += nsteps * j.dt
+
+    def set_dt(self, index, dt, stream=None):
+        job = self.jobs[index]
+        job.dt = float(dt)
+        _lib.call("mlv_batch_set_dt", self.handle, int(index), float(dt),
+                  _lib.stream_handle(stream))
+
+    def close(self):
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            _lib.lib().mlv_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def grid_points(self):
+        return sum(j.nx * j.nz for j in self.jobs)
+
+
+def cfl_dt(job, dt, cutoff=0.5):
+    """The reference's CFL rule (Integrator.py:35-44) on a job's physical
+    velocities (needs ``want_phys`` and a step run with write_phys)."""
+    from . import hip
+
+    nf = job.nfield
+    ux, uz = job.phys[nf], job.phys[nf + 1]
+    dx, dz = job.lx / job.nx, job.lz / job.nz
+    with np.errstate(divide="ignore", invalid="ignore"):
+        c = min(np.float64(dx) / np.float64(hip.max(ux)), np.float64(dz) / np.float64(hip.max(uz)))
+    if dt > c or math.isnan(c):
+        raise Exception("CFL condition breached")
+    while dt > cutoff * c:
+        dt = dt * 0.9
+    return dt
+
+
+def run(batch, nsteps, cfl_cadence=10, cutoff=0.5, on_cfl=None):
+    """Advance every job ``nsteps`` loops with the reference's CFL ticker
+    (Simulation.py:99-109, Ticker.py:14-25): dt is checked after loops 1,
+    1+cadence, ... from that loop's velocities and applies from the next
+    loop.  Jobs need ``want_phys``; cfl_cadence=0 disables the check."""
+    done = 0
+    next_check = 1
+    while done < nsteps:
+        if cfl_cadence and done + 1 == next_check:
+            batch.step(1, write_phys=True)
+            done += 1
+            for i, job in enumerate(batch.jobs):
+                new = cfl_dt(job, job.dt, cutoff)
+                if new != job.dt:
+                    batch.set_dt(i, new)
+                if on_cfl is not None:
+                    on_cfl(i, job)
+            next_check += cfl_cadence
+            continue
+        k = (next_check - 1 - done) if cfl_cadence else nsteps - done
+        k = min(max(k, 1), nsteps - done)
+        batch.step(k)
+        done += k
+    return done

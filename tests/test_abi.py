@@ -1,0 +1,76 @@
+"""The C ABI without a GPU: the library loads, exports every symbol the
+header declares, the ctypes mirror of mlv_job_desc matches the C layout,
+and argument validation fails cleanly before touching the device."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "melvin_hip.h")
+
+
+def _lib():
+    import __graft_entry__
+
+    m = __graft_entry__.load_package()
+    from melvin_amd import _lib as L  # noqa: F401
+
+    return m._lib
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\w[\w\s\*]*?\b(mlv_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib()
+    lib = L.lib()
+    declared = _declared()
+    assert len(declared) >= 20
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(declared) == set(L.EXPORTS)
+    assert lib.mlv_abi_version() == 1
+
+
+def test_job_desc_layout_matches_c(tmp_path):
+    L = _lib()
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "melvin_hip.h"\n'
+        "int main(void){printf(\"%zu %zu %zu %zu %zu\\n\", sizeof(mlv_job_desc),"
+        " offsetof(mlv_job_desc, dt), offsetof(mlv_job_desc, state),"
+        " offsetof(mlv_job_desc, phys), offsetof(mlv_job_desc, parity));return 0;}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    vals = [int(v) for v in subprocess.check_output([str(exe)]).split()]
+    J = L.JobDesc
+    assert vals == [ctypes.sizeof(J), J.dt.offset, J.state.offset, J.phys.offset,
+                    J.parity.offset]
+
+
+def test_argument_errors_are_reported():
+    L = _lib()
+    lib = L.lib()
+    assert lib.mlv_plan_create(None, 32, 32, 1.0, 1.0, 0, 1, 2) == L.MLV_ERR_ARG
+    assert b"NULL" in lib.mlv_last_error()
+    h = ctypes.c_void_p()
+    assert lib.mlv_plan_create(ctypes.byref(h), 48, 32, 1.0, 1.0, 0, 1, 2) == L.MLV_ERR_UNSUPPORTED
+    assert b"power of two" in lib.mlv_last_error()
+    assert lib.mlv_plan_create(ctypes.byref(h), 32, 32, 1.0, 1.0, 0, 1, 3) == L.MLV_ERR_ARG
+    assert lib.mlv_plan_create(ctypes.byref(h), 32, 32, -1.0, 1.0, 0, 1, 2) == L.MLV_ERR_ARG
+    with pytest.raises(NotImplementedError):
+        L.call("mlv_plan_create", ctypes.byref(h), 32, 33, 1.0, 1.0, 0, 1, 2)
+    assert lib.mlv_to_physical(None, None, None, None) == L.MLV_ERR_ARG
+    assert lib.mlv_batch_step(None, 1, 0, None) == L.MLV_ERR_ARG
+    assert lib.mlv_batch_steps(None) == -1
+    n = ctypes.c_size_t()
+    assert lib.mlv_job_work_bytes(9, 1, 32, 32, 0, ctypes.byref(n)) == L.MLV_ERR_ARG
+    assert lib.mlv_job_work_bytes(L.PROB_VORTICITY, 1, 32, 32, 0, ctypes.byref(n)) == 0
+    # TGV channels: w, ux, uz + Px, Pz -> 5 * nx * nm complex64
+    assert n.value == 5 * 32 * 10 * 8

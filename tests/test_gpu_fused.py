@@ -1,0 +1,150 @@
+"""The fused batched step (three HIP kernels per stage over a job table)
+against the reference's golden trajectories and the FP64 oracle, the
+alone-versus-in-a-mix identity, and the precision form."""
+import numpy as np
+import pytest
+
+from conftest import golden, rel
+from oracle import melvin_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _job(mlv, problem, params, dtype="f64", **kw):
+    p = O.make_params(params)
+    return mlv.fused.Job(problem, p.nx, p.nz, p.lx, p.lz, dtype=dtype, dt=p.initial_dt,
+                         ab_order=p.integrator_order, fd_order=p.spatial_derivative_order,
+                         alpha=p.alpha, **kw)
+
+
+def _golden_jobs(mlv, dtype="f64"):
+    """(job, check) pairs for every fully spectral golden trajectory."""
+    out = []
+    g, params, _ = golden("tgv_cfl")
+    j = _job(mlv, "vorticity", params, dtype, coeff=[0.05], want_psi=True, want_phys=True)
+    j.set_state([g["w0_spec"]])
+    out.append((j, len(g["dts"]), lambda j, g=g: [(j.state()[0], g["w_spec"]),
+                                                  (j.psi.cpu().numpy(), g["psi_spec"])]))
+    g, params, _ = golden("kh_ab4")
+    j = _job(mlv, "vorticity", params, dtype, coeff=[1e-3, 1e-3], want_phys=True)
+    j.set_state([g["w0_spec"], g["c0_spec"]])
+    out.append((j, len(g["dts"]), lambda j, g=g: [(j.state()[0], g["w_spec"]),
+                                                  (j.state()[1], g["c_spec"])]))
+    g, params, _ = golden("vort_o4")
+    j = _job(mlv, "vorticity", params, dtype, coeff=[2e-3], want_phys=True)
+    j.set_state([g["w0_spec"]])
+    out.append((j, len(g["dts"]), lambda j, g=g: [(j.state()[0], g["w_spec"])]))
+    g, params, p = golden("ddc")
+    j = _job(mlv, "ddc", params, dtype, coeff=[p.Pr, 1.0, p.tau], param=[p.Pr, p.R0],
+             want_phys=True)
+    j.set_state([g["w0_spec"], g["t0_spec"], g["s0_spec"]])
+    out.append((j, 15, lambda j, g=g: [(j.state()[0], g["w_spec"]), (j.state()[1], g["t_spec"]),
+                                       (j.state()[2], g["s_spec"])]))
+    g, params, p = golden("mhd")
+    j = _job(mlv, "mhd", params, dtype, coeff=[1 / p.Re, 1 / p.S], want_phys=True)
+    j.set_state([g["w0_spec"], g["j0_spec"]])
+    out.append((j, 15, lambda j, g=g: [(j.state()[0], g["w_spec"]), (j.state()[1], g["j_spec"])]))
+    return out
+
+
+def test_fused_single_jobs_vs_golden(mlv):
+    for job, n, check in _golden_jobs(mlv):
+        b = mlv.fused.Batch([job])
+        mlv.fused.run(b, n)
+        for got, want in check(job):
+            assert rel(got, want) < 1e-12, job.problem
+        b.close()
+
+
+def test_fused_cfl_matches_reference_dt(mlv):
+    g, params, _ = golden("tgv_cfl")
+    j = _job(mlv, "vorticity", params, coeff=[0.05], want_phys=True)
+    j.set_state([g["w0_spec"]])
+    b = mlv.fused.Batch([j])
+    mlv.fused.run(b, len(g["dts"]))
+    assert j.dt == float(g["final_dt"])
+    assert j.t == pytest.approx(float(g["t"]), rel=1e-15)
+
+
+def test_mixed_batch_equals_alone(mlv):
+    """Every job's result is identical alone and inside a mixed batch."""
+    alone = []
+    for job, n, _ in _golden_jobs(mlv):
+        b = mlv.fused.Batch([job])
+        b.step(12)
+        alone.append(job.state())
+        b.close()
+    jobs = [j for j, _, _ in _golden_jobs(mlv)]
+    b = mlv.fused.Batch(jobs)
+    b.step(12)
+    for j, ref in zip(jobs, alone):
+        for x, y in zip(j.state(), ref):
+            np.testing.assert_array_equal(x, y)
+
+
+def test_fused_physical_outputs(mlv):
+    g, params, p = golden("kh_ab4")
+    j = _job(mlv, "vorticity", params, coeff=[1e-3, 1e-3], want_phys=True, want_psi=True)
+    j.set_state([g["w0_spec"], g["c0_spec"]])
+    b = mlv.fused.Batch([j])
+    b.step(1, write_phys=True)
+    w, c, ux, uz = j.physical()
+    psi, oux, ouz, _, _ = O.velocity_from_vorticity(p, g["w0_spec"])
+    assert rel(w, O.to_physical(p, g["w0_spec"])) < 1e-13
+    assert rel(c, O.to_physical(p, g["c0_spec"])) < 1e-13
+    assert rel(ux, oux) < 1e-13
+    assert rel(uz, ouz) < 1e-13
+    assert rel(j.psi.cpu().numpy(), psi) < 1e-13
+
+
+@pytest.mark.parametrize("nx,nz", [(64, 32), (128, 128), (256, 64), (2048, 16), (16, 1024)])
+def test_fused_sizes_vs_oracle(mlv, nx, nz):
+    """Non-square grids of many sizes, 3 steps, AB2, FD order 2."""
+    params = dict(nx=nx, nz=nz, lx=2.0, lz=1.5, initial_dt=1e-4)
+    p = O.make_params(params)
+    rng = np.random.default_rng(nx + 3 * nz)
+    w0 = O.to_spectral(p, rng.standard_normal((nx, nz)))
+    j = _job(mlv, "vorticity", params, coeff=[1e-3])
+    j.set_state([w0])
+    b = mlv.fused.Batch([j])
+    b.step(3)
+    st = O.new_state(p, ("w",))
+    st["w"] = w0.copy()
+    for _ in range(3):
+        O.step_vorticity(p, st, 1e-3)
+    assert rel(j.state()[0], st["w"]) < 1e-12
+
+
+def test_config1_taylor_green_256_fp64(mlv):
+    """Taylor-Green 256^2 on [0, 2pi]^2, nu = 1e-2, AB2 semi-implicit, dt = 1e-3,
+    1000 steps: <= 1e-12 relative L2 in omega and psi against the oracle, and
+    the (1,1) mode decays as exp(-2 nu t) (the theta-scheme error at this dt
+    is ~1e-9)."""
+    params = dict(nx=256, nz=256, lx=2 * np.pi, lz=2 * np.pi, initial_dt=1e-3)
+    p = O.make_params(params)
+    w0 = O.ic_taylor_green(p, seed=0, amp=1e-6, kmax=8)
+    j = _job(mlv, "vorticity", params, coeff=[1e-2], want_psi=True, want_phys=True)
+    j.set_state([w0])
+    b = mlv.fused.Batch([j])
+    mlv.fused.run(b, 1000)
+    st = O.new_state(p, ("w",))
+    st["w"] = w0.copy()
+    O.run(p, st, lambda p, s: O.step_vorticity(p, s, 1e-2), 1000)
+    assert j.dt == st["dt"] == 1e-3
+    assert rel(j.state()[0], st["w"]) <= 1e-12
+    assert rel(j.psi.cpu().numpy(), st["psi"]) <= 1e-12
+    amp = j.state()[0][1, 1] / w0[1, 1]
+    assert abs(amp - np.exp(-2 * 1e-2 * 1.0)) < 1e-6
+
+
+def test_precision_form_fp32_close_to_fp64(mlv):
+    g, params, _ = golden("kh_ab4")
+    res = {}
+    for dt in ("f64", "f32"):
+        j = _job(mlv, "vorticity", params, dt, coeff=[1e-3, 1e-3])
+        j.set_state([g["w0_spec"], g["c0_spec"]])
+        b = mlv.fused.Batch([j])
+        b.step(50)
+        res[dt] = j.state()
+    assert rel(res["f32"][0], res["f64"][0]) < 1e-5
+    assert rel(res["f32"][1], res["f64"][1]) < 1e-5
