@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Throughput of the batched FP32 fused step at 1024^2 against the plain
+FP64 path, on one MI355X.
+
+Workload: the Kelvin-Helmholtz shear layer of the precision configuration
+-- doubly periodic [0,2]x[0,2], 1024x1024, omega plus a passive scalar,
+u = tanh((z-0.5)/0.05) - tanh((z-1.5)/0.05), seeded omega noise 1e-3,
+Re = 1e4, AB4 with semi-implicit diffusion, dt = 2e-4 -- run as a batch of
+independent simulations (distinct noise seeds) in FP32.  A "step" advances
+every simulation of the batch by one timestep; `value` counts grid points x
+steps over the whole batch.  The same workload (one simulation) in FP64 is
+timed after it as the plain path, and the FP32 result of simulation 0 is
+compared with the FP64 one after the timed steps.
+
+Printed: one JSON line with the metric, the dominant kernel's measured
+bandwidth against the HBM roofline, and a CPU baseline (the NumPy FP64
+oracle on a bounded sample, one core).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid-pt-steps/s batched FP32 vs plain FP64 @ rel err ≤1e-5; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+NX = NZ = 1024
+LX = LZ = 2.0
+RE = 1e4
+DT = 2e-4
+Q = 4
+
+
+def kh_initial(seed):
+    from oracle import melvin_oracle as O
+
+    p = O.make_params(dict(nx=NX, nz=NZ, lx=LX, lz=LZ, initial_dt=DT, integrator_order=Q))
+    w, c = O.ic_kelvin_helmholtz(p, seed=seed, amp=1e-3)
+    return p, O.to_spectral(p, w), O.to_spectral(p, c)
+
+
+def kernel_bytes(nx, nz, nf, q, cbytes, carry=False):
+    """Algorithmic HBM bytes of one launch of each fused kernel for one
+    simulation with nf advected fields: the minimum reads and writes of its
+    pass (each operand once).  x_forward_update reads Px, Pz, the state and
+    q-1 history slots, writes the state and one history slot, and with the
+    single-precision carry also reads and writes that."""
+    nn, nm = (nx - 1) // 3, (nz - 1) // 3
+    S = (2 * nn + 1) * nm      # compact spectral
+    I = nx * nm                # x-pass intermediate
+    nch = nf + 2               # fields + ux + uz
+    a = (nf * S + nch * I) * cbytes
+    b = (nch * I + 2 * nf * I) * cbytes
+    c = (2 * nf * I + nf * S * (1 + (q - 1) + 2 + (2 if carry else 0))) * cbytes
+    return {"x_inverse": a, "z_pass": b, "x_forward_update": c}
+
+
+def cpu_baseline(seconds=12.0):
+    """The oracle's FP64 KH step (NumPy/pocketfft, one thread) on the host."""
+    from oracle import melvin_oracle as O
+
+    try:
+        old = os.sched_getaffinity(0)
+        os.sched_setaffinity(0, {min(old)})
+    except (AttributeError, OSError):
+        old = None
+    try:
+        p, w, c = kh_initial(0)
+        st = O.new_state(p, ("w", "c"), dt=DT)
+        st["w"], st["c"] = w, c
+        body = lambda p, s: O.step_vorticity(p, s, 1.0 / RE, ("c",))  # noqa: E731
+        body(p, st)
+        n = 0
+        t0 = time.perf_counter()
+        while True:
+            body(p, st)
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= seconds and n >= 3:
+                break
+        return {"value": NX * NZ * n / el, "unit": "grid-pt-steps/s", "cores": 1,
+                "kind": "port",
+                "sample": f"NumPy FP64 oracle, Kelvin-Helmholtz 1024^2 omega+scalar AB4, "
+                          f"{n} steps after 1 warm-up ({el:.1f} s), 1 simulation, one core"}
+    finally:
+        if old is not None:
+            os.sched_setaffinity(0, old)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--fp64-steps", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not "
+                          "supported (independent simulations would shard by job table "
+                          "with no exchange)"}))
+        sys.exit(2)
+
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
+
+    import torch
+
+    import __graft_entry__
+
+    m = __graft_entry__.load_package()
+    fused = m.fused
+    torch.cuda.init()
+    print(f"[bench] device {torch.cuda.get_device_name(0)}; batch {args.batch} x "
+          f"{NX}x{NZ} FP32; {args.warmup} warm-up + {args.steps} timed steps",
+          file=sys.stderr, flush=True)
+
+    def make_jobs(n, dtype):
+        jobs = []
+        for s in range(n):
+            _, w, c = kh_initial(s)
+            j = fused.Job("vorticity", NX, NZ, LX, LZ, dtype=dtype, dt=DT,
+                          coeff=[1.0 / RE, 1.0 / RE], ab_order=Q)
+            j.set_state([w, c])
+            jobs.append(j)
+        return jobs
+
+    def timed(batch, steps, warmup):
+        batch.step(warmup)
+        torch.cuda.synchronize()
+        m._lib.call("mlv_batch_profile", batch.handle, 1)
+        t0 = time.perf_counter()
+        batch.step(steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        import ctypes
+
+        ms = (ctypes.c_double * 3)()
+        n = (ctypes.c_longlong * 3)()
+        m._lib.call("mlv_batch_profile_read", batch.handle, ms, n)
+        m._lib.call("mlv_batch_profile", batch.handle, 0)
+        return el, [ms[k] / max(n[k], 1) for k in range(3)], [n[k] for k in range(3)]
+
+    # batched FP32 (the headline)
+    jobs32 = make_jobs(args.batch, "f32")
+    b32 = fused.Batch(jobs32)
+    el32, kms32, nl32 = timed(b32, args.steps, args.warmup)
+    pts = args.batch * NX * NZ
+    value = pts * args.steps / el32
+    w32 = jobs32[0].state()[0]
+    print(f"[bench] fp32 batched: {el32 * 1e3 / args.steps:.3f} ms/step, {value:.3e} "
+          f"grid-pt-steps/s; kernel ms {kms32}", file=sys.stderr, flush=True)
+
+    # plain FP64, one simulation, same number of steps from the same start
+    steps64 = args.fp64_steps or args.steps
+    job64 = make_jobs(1, "f64")[0]
+    b64 = fused.Batch([job64])
+    el64, kms64, _ = timed(b64, steps64, args.warmup)
+    value64 = NX * NZ * steps64 / el64
+    w64 = job64.state()[0]
+    err = float(np.linalg.norm(w32 - w64) / np.linalg.norm(w64)) if steps64 == args.steps else None
+    print(f"[bench] fp64 plain: {el64 * 1e3 / steps64:.3f} ms/step, {value64:.3e} "
+          f"grid-pt-steps/s; fp32 vs fp64 omega rel L2 {err}", file=sys.stderr, flush=True)
+
+    names = ["x_inverse", "z_pass", "x_forward_update"]
+    kb = kernel_bytes(NX, NZ, 2, Q, 8, carry=jobs32[0].carry is not None)
+    dom = int(np.argmax(kms32))
+    launch_bytes = kb[names[dom]] * args.batch
+    achieved = launch_bytes / (kms32[dom] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "grid-pt-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el32 * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"Kelvin-Helmholtz shear layer {NX}x{NZ} on [0,2]^2, omega + passive "
+                        f"scalar, AB4 semi-implicit, Re=1e4, dt=2e-4, batch of {args.batch} "
+                        f"independent simulations",
+            "batch": args.batch, "nx": NX, "nz": NZ, "precision": "fp32",
+            "parallelism": "single GPU",
+        },
+        "fp64_plain": {"value": value64, "ms_per_step": el64 * 1e3 / steps64,
+                       "steps": steps64, "kernel_ms": kms64},
+        "ratio_batched_fp32_vs_plain_fp64": value / value64,
+        "rel_err_fp32_vs_fp64_omega": err,
+        "kernel_ms_per_launch": dict(zip(names, kms32)),
+        "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_launch": launch_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    b32.close()
+    b64.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -162,6 +162,8 @@ typedef struct mlv_job_desc {
   int nfield;
   void* state[2][4];   /* ping-pong spectral state per field */
   void* hist[4];       /* AB ring per field: ab_order x spectral */
+  void* comp[4];       /* F32 only, optional: per-field Kahan carry of the
+                          state (spectral, zero-initialised), or NULL */
   void* work;          /* mlv_job_work_bytes() bytes */
   void* psi;           /* optional spectral psi of the step, or NULL */
   void* phys[8];       /* optional physical outputs (problem order), or NULL */
@@ -175,10 +177,22 @@ int mlv_batch_destroy(mlv_batch* batch);
 /* Run nsteps steps; write_phys = 1 also writes the physical fields of the
  * last of them.  Each step reads dt from the job table. */
 int mlv_batch_step(mlv_batch* batch, int nsteps, int write_phys, void* stream);
+/* Same as mlv_batch_step without physical outputs, replayed from a HIP
+ * graph captured on first use (re-captured when nsteps changes).  nsteps
+ * must be a multiple of 4 and the batch's step count a multiple of 4 on
+ * entry. */
+int mlv_batch_graph_step(mlv_batch* batch, int nsteps, void* stream);
 /* Set one job's dt for the following steps (stream-ordered). */
 int mlv_batch_set_dt(mlv_batch* batch, int job, double dt, void* stream);
 /* Steps completed since creation (host count). */
 long long mlv_batch_steps(const mlv_batch* batch);
+/* Kernel timing (no reference counterpart; measurement only): while enabled,
+ * every step brackets its three kernels with HIP events on the step's
+ * stream.  Enabling or disabling resets the totals. */
+int mlv_batch_profile(mlv_batch* batch, int enable);
+/* Milliseconds and launch counts of the x-inverse, z and x-forward/update
+ * kernels since profiling was enabled (synchronises with the events). */
+int mlv_batch_profile_read(mlv_batch* batch, double* ms, long long* launches);
 
 #ifdef __cplusplus
 }

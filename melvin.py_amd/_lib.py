@@ -34,8 +34,8 @@ EXPORTS = (
     "mlv_plan_info", "mlv_to_physical", "mlv_to_spectral", "mlv_spec_op",
     "mlv_fd_op", "mlv_flux_divergence", "mlv_poisson", "mlv_update",
     "mlv_reduce", "mlv_combine", "mlv_velocity", "mlv_job_work_bytes", "mlv_batch_create",
-    "mlv_batch_destroy", "mlv_batch_step", "mlv_batch_set_dt",
-    "mlv_batch_steps",
+    "mlv_batch_destroy", "mlv_batch_step", "mlv_batch_graph_step", "mlv_batch_set_dt",
+    "mlv_batch_steps", "mlv_batch_profile", "mlv_batch_profile_read",
 )
 
 
@@ -61,6 +61,7 @@ class JobDesc(ctypes.Structure):
         ("nfield", ctypes.c_int),
         ("state", (ctypes.c_void_p * 4) * 2),
         ("hist", ctypes.c_void_p * 4),
+        ("comp", ctypes.c_void_p * 4),
         ("work", ctypes.c_void_p),
         ("psi", ctypes.c_void_p),
         ("phys", ctypes.c_void_p * 8),
@@ -96,8 +97,11 @@ def _declare(lib):
         "mlv_batch_create": ([pp, i, ctypes.POINTER(JobDesc), i], i),
         "mlv_batch_destroy": ([vp], i),
         "mlv_batch_step": ([vp, i, i, vp], i),
+        "mlv_batch_graph_step": ([vp, i, vp], i),
         "mlv_batch_set_dt": ([vp, i, d, vp], i),
         "mlv_batch_steps": ([vp], ctypes.c_longlong),
+        "mlv_batch_profile": ([vp, i], i),
+        "mlv_batch_profile_read": ([vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_longlong)], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -206,7 +206,10 @@ int mlv_plan_create(mlv_plan** out, int nx, int nz, double lx, double lz, int di
             "upload thomas");
   }
   HIP_TRY(hipMalloc((void**)&p->red_part, reduce_partials_bytes()), "hipMalloc partials");
-  const size_t lds_need = dtype == MLV_F64 ? fft_lds_bytes<double>(lnx) : fft_lds_bytes<float>(lnx);
+  size_t lds_need = dtype == MLV_F64 ? fft_lds_bytes<double>(lnx) : fft_lds_bytes<float>(lnx);
+  if (lnz > 0)
+    lds_need = std::max(lds_need, dtype == MLV_F64 ? fft_lds_bytes_row<double>(lnz)
+                                                   : fft_lds_bytes_row<float>(lnz));
   if (lds_need > max_lds_per_block(p->device)) {
     mlv_plan_destroy(p);
     return fail(MLV_ERR_UNSUPPORTED, "transform does not fit the LDS of this device");
@@ -485,11 +488,49 @@ struct BatchHost {
   int dtype = 0, njobs = 0, device = 0, nstage = 1;
   void* jobs_dev = nullptr;
   BatchLaunch bl{};
-  long long* step_dev = nullptr;
   long long steps = 0;
   const void* tw = nullptr;
   std::vector<void*> allocs;
+  // optional kernel timing: 4 events per stage and step (before A, B, C, after C)
+  // captured sequence of graph_steps steps (see mlv_batch_graph_step)
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int graph_steps = 0;
+  bool prof = false;
+  std::vector<hipEvent_t> events;
+  size_t used = 0;
+  double ms[3] = {0, 0, 0};
+  long long launches[3] = {0, 0, 0};
 };
+
+hipEvent_t* prof_slots(BatchHost* b) {
+  if (!b->prof) return nullptr;
+  while (b->events.size() < b->used + 4) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    b->events.push_back(e);
+  }
+  hipEvent_t* r = &b->events[b->used];
+  b->used += 4;
+  return r;
+}
+
+int prof_collect(BatchHost* b) {
+  if (b->used == 0) return MLV_OK;
+  HIP_TRY(hipEventSynchronize(b->events[b->used - 1]), "hipEventSynchronize");
+  for (size_t i = 0; i + 3 < b->used; i += 4) {
+    for (int k = 0; k < 3; ++k) {
+      float t = 0.f;
+      HIP_TRY(hipEventElapsedTime(&t, b->events[i + k], b->events[i + k + 1]),
+              "hipEventElapsedTime");
+      b->ms[k] += t;
+      b->launches[k] += 1;
+    }
+  }
+  b->used = 0;
+  return MLV_OK;
+}
 
 int problem_fields(int prob) {
   switch (prob) {
@@ -564,6 +605,7 @@ int build_job(const mlv_job_desc& d, JobDev<T>& J, int j) {
   for (int b = 0; b < 2; ++b)
     for (int f = 0; f < nf; ++f) J.state[b][f] = (cx<T>*)d.state[b][f];
   for (int f = 0; f < nf; ++f) J.hist[f] = (cx<T>*)d.hist[f];
+  for (int f = 0; f < nf; ++f) J.comp[f] = sizeof(T) == 4 ? (cx<T>*)d.comp[f] : nullptr;
   J.work = (cx<T>*)d.work;
   J.psi = (cx<T>*)d.psi;
   for (int i = 0; i < kMaxPhys; ++i) J.phys[i] = (T*)d.phys[i];
@@ -684,10 +726,11 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
       }
     }
     const int slx = dtype == MLV_F64 ? slots_for<double>(lognx) : slots_for<float>(lognx);
-    const int slz = dtype == MLV_F64 ? slots_for<double>(lognz) : slots_for<float>(lognz);
+    const int slz = dtype == MLV_F64 ? slots_row<double>(lognz) : slots_row<float>(lognz);
     const size_t ldsx = dtype == MLV_F64 ? fft_lds_bytes<double>(lognx) : fft_lds_bytes<float>(lognx);
-    const size_t ldsz = dtype == MLV_F64 ? fft_lds_bytes<double>(lognz) : fft_lds_bytes<float>(lognz);
-    if (ldsx > lds_max || ldsz > lds_max || slx < 2) {
+    const size_t ldsz =
+        dtype == MLV_F64 ? fft_lds_bytes_row<double>(lognz) : fft_lds_bytes_row<float>(lognz);
+    if (ldsx > lds_max || ldsz > lds_max || slx < 1) {
       rc = fail(MLV_ERR_UNSUPPORTED, "job " + std::to_string(j) + ": grid too large for LDS");
       break;
     }
@@ -703,7 +746,7 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
         }
         const int rb = brows(nin_v, neq_v, slz);
         a = nA[s] * ((nm + slx - 1) / slx);
-        c = neq[s] * ((nm + slx / 2 - 1) / (slx / 2));
+        c = neq[s] * ((nm + slx - 1) / slx);
         bb = (nx + rb - 1) / rb;
         bv = bvar[s];
         bl.ldsB[bv] = std::max(bl.ldsB[bv], ldsz);
@@ -733,7 +776,6 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
     return MLV_OK;
   };
   rc = dalloc(host.size(), &b->jobs_dev);
-  if (!rc) rc = dalloc(sizeof(long long), (void**)&b->step_dev);
   for (int s = 0; s < 2 && !rc; ++s) {
     bl.nA[s] = pA[s][njobs];
     bl.nC[s] = pC[s][njobs];
@@ -749,8 +791,8 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
     if (e != hipSuccess) rc = hip_fail(e, "upload jobs");
   }
   if (!rc) {
-    hipError_t e = hipMemset(b->step_dev, 0, sizeof(long long));
-    if (e != hipSuccess) rc = hip_fail(e, "clear step counter");
+    hipError_t e = dtype == MLV_F64 ? fused_prepare<double>(bl) : fused_prepare<float>(bl);
+    if (e != hipSuccess) rc = hip_fail(e, "kernel attributes");
   }
   if (rc) {
     for (void* p : b->allocs) (void)hipFree(p);
@@ -766,32 +808,96 @@ int mlv_batch_destroy(mlv_batch* bp) {
   BatchHost* b = reinterpret_cast<BatchHost*>(bp);
   (void)hipDeviceSynchronize();
   for (void* p : b->allocs) (void)hipFree(p);
+  for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
   delete b;
   return MLV_OK;
 }
+
+int mlv_batch_profile(mlv_batch* bp, int enable) {
+  if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
+  BatchHost* b = reinterpret_cast<BatchHost*>(bp);
+  int rc = prof_collect(b);
+  if (rc) return rc;
+  b->prof = enable != 0;
+  for (int k = 0; k < 3; ++k) {
+    b->ms[k] = 0;
+    b->launches[k] = 0;
+  }
+  return MLV_OK;
+}
+
+int mlv_batch_profile_read(mlv_batch* bp, double* ms, long long* launches) {
+  if (!bp || !ms || !launches) return fail(MLV_ERR_ARG, "NULL argument");
+  BatchHost* b = reinterpret_cast<BatchHost*>(bp);
+  int rc = prof_collect(b);
+  if (rc) return rc;
+  for (int k = 0; k < 3; ++k) {
+    ms[k] = b->ms[k];
+    launches[k] = b->launches[k];
+  }
+  return MLV_OK;
+}
+
+static hipError_t enqueue_steps(BatchHost* b, int nsteps, int write_phys, long long first,
+                                hipStream_t s, bool timed);
 
 int mlv_batch_step(mlv_batch* bp, int nsteps, int write_phys, void* stream) {
   if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
   if (nsteps < 0) return fail(MLV_ERR_ARG, "nsteps < 0");
   BatchHost* b = reinterpret_cast<BatchHost*>(bp);
-  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = enqueue_steps(b, nsteps, write_phys, b->steps, (hipStream_t)stream, b->prof);
+  if (e != hipSuccess) return hip_fail(e, "fused step");
+  b->steps += nsteps;
+  return MLV_OK;
+}
+
+static hipError_t enqueue_steps(BatchHost* b, int nsteps, int write_phys, long long first,
+                                hipStream_t s, bool timed) {
   for (int k = 0; k < nsteps; ++k) {
     const int wp = (write_phys && k == nsteps - 1) ? 1 : 0;
+    const int step = (int)((first + k) & 3);
     for (int st = 0; st < b->nstage; ++st) {
+      hipEvent_t* ev = timed ? prof_slots(b) : nullptr;
       hipError_t e =
           b->dtype == MLV_F64
               ? fused_stage<double>((const JobDev<double>*)b->jobs_dev, b->bl, st, st == 0 ? wp : 0,
-                                    b->step_dev, (const cx<double>*)b->tw, s)
+                                    step, (const cx<double>*)b->tw, s, ev)
               : fused_stage<float>((const JobDev<float>*)b->jobs_dev, b->bl, st, st == 0 ? wp : 0,
-                                   b->step_dev, (const cx<float>*)b->tw, s);
-      if (e != hipSuccess) return hip_fail(e, "fused stage");
+                                   step, (const cx<float>*)b->tw, s, ev);
+      if (e != hipSuccess) return e;
     }
-    hipError_t e = b->dtype == MLV_F64
-                       ? fused_advance<double>(nullptr, b->njobs, b->step_dev, nullptr, s)
-                       : fused_advance<float>(nullptr, b->njobs, b->step_dev, nullptr, s);
-    if (e != hipSuccess) return hip_fail(e, "advance");
-    b->steps += 1;
   }
+  return hipSuccess;
+}
+
+int mlv_batch_graph_step(mlv_batch* bp, int nsteps, void* stream) {
+  if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
+  BatchHost* b = reinterpret_cast<BatchHost*>(bp);
+  if (nsteps <= 0 || (nsteps & 3)) return fail(MLV_ERR_ARG, "graph steps must be a positive multiple of 4");
+  if (b->steps & 3)
+    return fail(MLV_ERR_ARG, "graph replay needs the batch's step count at a multiple of 4");
+  if (!b->graph_exec || b->graph_steps != nsteps) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    if (!b->cap_stream)
+      HIP_TRY(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking), "capture stream");
+    HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    hipError_t e = enqueue_steps(b, nsteps, 0, 0, b->cap_stream, false);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->cap_stream, &g);
+    if (e != hipSuccess) return hip_fail(e, "capture");
+    if (e2 != hipSuccess) return hip_fail(e2, "end capture");
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0), "instantiate");
+    b->graph_steps = nsteps;
+  }
+  HIP_TRY(hipGraphLaunch(b->graph_exec, (hipStream_t)stream), "graph launch");
+  b->steps += nsteps;
   return MLV_OK;
 }
 

@@ -18,7 +18,9 @@
 namespace mlv {
 
 __host__ __device__ constexpr int fft_pad(int i) { return i + (i >> 4); }
-__host__ __device__ constexpr int fft_ldsn(int n) { return n + (n >> 4); }
+// Slot stride: the padded length plus one element, so that the same index
+// of consecutive slots falls on different LDS banks.
+__host__ __device__ constexpr int fft_ldsn(int n) { return n + (n >> 4) + 1; }
 
 // cos(2*pi*j/16), j = 0..15.
 __device__ constexpr double kCos16[16] = {
@@ -150,7 +152,18 @@ __host__ __device__ constexpr int fft_slots() {
   return (NT * P) >> LOGN;
 }
 
-// Points per thread used for a transform length: 16 up to 1024, 32 beyond.
-__host__ __device__ constexpr int fft_points(int logn) { return logn >= 11 ? 32 : 16; }
+// Points per thread.  Column passes (along x, strided) want enough slots per
+// block that every row is read in runs of at least 64 bytes (8 float or 4
+// double complex values), so single precision uses 32 points from N = 1024.
+// Row passes read contiguous rows and keep 16 points up to N = 1024, which
+// leaves more blocks and a smaller register file.
+template <typename T>
+__host__ __device__ constexpr int fft_points_col(int logn) {
+  return sizeof(T) == 4 ? (logn >= 10 ? 32 : 16) : (logn >= 11 ? 32 : 16);
+}
+template <typename T>
+__host__ __device__ constexpr int fft_points_row(int logn) {
+  return logn >= 11 ? 32 : 16;
+}
 
 }  // namespace mlv

@@ -15,9 +15,7 @@ __device__ __forceinline__ int find_job(const int* __restrict__ pref, int njobs,
   return lo;
 }
 
-__device__ __forceinline__ int ring(long long c0, long long step, int q) {
-  return (int)((c0 + step) % q);
-}
+__device__ __forceinline__ int ring(int c0, int step, int q) { return (c0 + step) % q; }
 
 // --------------------------------------------------------------------------
 // Kernel A: spectral prologue + inverse x pass
@@ -26,7 +24,7 @@ template <typename T, int LOGN>
 __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S, int local,
                                            int par, const cx<T>* __restrict__ tw, cx<T>* lds) {
   constexpr int N = 1 << LOGN;
-  constexpr int P = fft_points(LOGN);
+  constexpr int P = fft_points_col<T>(LOGN);
   constexpr int TC = fft_slots<LOGN, kNT, P>();
   constexpr int LDSN = fft_ldsn(N);
   const int nn = J.nn, nm = J.nm, rows = J.rows;
@@ -81,14 +79,13 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
 template <typename T>
 __global__ void __launch_bounds__(kNT) k_stepA(const JobDev<T>* __restrict__ jobs,
                                                const int* __restrict__ pref, int njobs, int stage,
-                                               const long long* __restrict__ step,
-                                               const cx<T>* __restrict__ tw) {
+                                               int step, const cx<T>* __restrict__ tw) {
   extern __shared__ char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int j = find_job(pref, njobs, blockIdx.x);
   const JobDev<T>& J = jobs[j];
   const int local = blockIdx.x - pref[j];
-  const int par = (int)((J.par0 + *step) & 1);
+  const int par = (J.par0 + step) & 1;
   switch (J.lognx) {
 #define X(L)                                                       \
   case L:                                                          \
@@ -110,7 +107,7 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
   using Pg = BProg<V>;
   constexpr int NIN = Pg::NIN, NEQ = Pg::NEQ;
   constexpr int N = 1 << LOGN;
-  constexpr int P = fft_points(LOGN);
+  constexpr int P = fft_points_row<T>(LOGN);
   constexpr int SL = fft_slots<LOGN, kNT, P>();
   constexpr int LDSN = fft_ldsn(N);
   if constexpr (!bfits(NIN, NEQ, SL)) {
@@ -229,9 +226,8 @@ template <typename T, int LOGN>
 __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S, int local, int cur,
                                            int par, const cx<T>* __restrict__ tw, cx<T>* lds) {
   constexpr int N = 1 << LOGN;
-  constexpr int P = fft_points(LOGN);
-  constexpr int SL = fft_slots<LOGN, kNT, P>();
-  constexpr int TC = SL / 2;
+  constexpr int P = fft_points_col<T>(LOGN);
+  constexpr int TC = fft_slots<LOGN, kNT, P>();
   constexpr int LDSN = fft_ldsn(N);
   const int nn = J.nn, nm = J.nm, rows = J.rows;
   const size_t chs = (size_t)N * nm;
@@ -243,18 +239,35 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   const int ncol = min(TC, nm - c0);
   const cx<T>* __restrict__ pxc = J.work + (size_t)pchan(J.pbase, eqi, 0) * chs;
   const cx<T>* __restrict__ pzc = J.work + (size_t)pchan(J.pbase, eqi, 1) * chs;
+  // One transform per column: the periodic x difference acts on the
+  // intermediate rows (it commutes with the z transform) and the z
+  // difference is its modified wavenumber, so
+  //   D = F_x[ FDx(Px) + i sz(kz) Pz ].
+  const int order = J.fd_order;
+  const T ihx = order == 2 ? (T)(1.0 / (2.0 * J.dx)) : (T)(1.0 / (3.0 * J.dx));
   for (int e = threadIdx.x; e < N * TC; e += kNT) {
     const int c = e % TC, x = e / TC;
-    cx<T> a = czero<T>(), b = czero<T>();
+    cx<T> v = czero<T>();
     if (c < ncol) {
-      a = pxc[(size_t)x * nm + c0 + c];
-      b = pzc[(size_t)x * nm + c0 + c];
+      const int m = c0 + c;
+      auto px = [&](int dxr) { return pxc[(size_t)((x + dxr) & (N - 1)) * nm + m]; };
+      cx<T> d;
+      if (order == 2) {
+        const cx<T> a = px(1), b = px(-1);
+        d = mk<T>((a.x - b.x) * ihx, (a.y - b.y) * ihx);
+      } else {
+        const cx<T> a2 = px(2), a1 = px(1), b1 = px(-1), b2 = px(-2);
+        d = mk<T>((T(-0.25) * a2.x + T(2) * a1.x - T(2) * b1.x + T(0.25) * b2.x) * ihx,
+                  (T(-0.25) * a2.y + T(2) * a1.y - T(2) * b1.y + T(0.25) * b2.y) * ihx);
+      }
+      const T sz = modk<T>(tw, m, J.lognz, order, J.dz);
+      const cx<T> z = pzc[(size_t)x * nm + m];
+      v = mk<T>(d.x - sz * z.y, d.y + sz * z.x);
     }
-    lds[(2 * c) * LDSN + fft_pad(x)] = a;
-    lds[(2 * c + 1) * LDSN + fft_pad(x)] = b;
+    lds[c * LDSN + fft_pad(x)] = v;
   }
   __syncthreads();
-  fft_lds<T, LOGN, -1, kNT, P>(lds, 2 * ncol, tw);
+  fft_lds<T, LOGN, -1, kNT, P>(lds, ncol, tw);
 
   const int q = J.q;
   const double dt = J.dt, alpha = J.alpha;
@@ -272,12 +285,8 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
     const int n = r <= nn ? r : r - rows;
     const int m = c0 + c;
     const size_t idx = (size_t)r * nm + m;
-    const cx<T> X = lds[(2 * c) * LDSN + fft_pad(x)];
-    const cx<T> Z = lds[(2 * c + 1) * LDSN + fft_pad(x)];
-    const T sx = modk<T>(tw, n, LOGN, J.fd_order, J.dx);
-    const T sz = modk<T>(tw, m, J.lognz, J.fd_order, J.dz);
-    // i * (sx X + sz Z) / (nx nz)
-    cx<T> rhs = mk<T>(-(sx * X.y + sz * Z.y) * scale, (sx * X.x + sz * Z.x) * scale);
+    const cx<T> D = lds[c * LDSN + fft_pad(x)];
+    cx<T> rhs = mk<T>(D.x * scale, D.y * scale);
     if (J.problem == MLV_PROB_DDC) {
       const double Pr = J.param[0], R0 = J.param[1];
       if (field == 0) {
@@ -320,10 +329,35 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
       }
     }
     const double lin = coeff * (J.fx2 * (double)(n * n) + J.fz2 * (double)(m * m));
-    const T num = (T)(1.0 + c1 * lin);
-    const T rden = (T)(1.0 / (1.0 - c2 * lin));
+    const double den = 1.0 - c2 * lin;
     const cx<T> v = sin_[idx];
-    cx<T> nv = mk<T>((num * v.x + (T)hab * abx) * rden, (num * v.y + (T)hab * aby) * rden);
+    cx<T> nv;
+    if constexpr (sizeof(T) == 8) {
+      // (1 + (1-a) dt L) v + AB, divided by (1 - a dt L)  (Integrator.py:58-63)
+      const T num = (T)(1.0 + c1 * lin);
+      const T rden = (T)(1.0 / den);
+      nv = mk<T>((num * v.x + (T)hab * abx) * rden, (num * v.y + (T)hab * aby) * rden);
+    } else {
+      // The same update written as an increment, v + (dt L v + AB)/(1 - a dt L),
+      // so the rounding falls on the small change, not on (1 + (1-a) dt L).
+      // With a carry array the addition is Kahan-compensated: increments
+      // below half an ulp of v are kept instead of being dropped every step.
+      const T a = (T)(dt * lin);
+      const T rden = (T)(1.0 / den);
+      const T ix = (a * v.x + (T)hab * abx) * rden, iy = (a * v.y + (T)hab * aby) * rden;
+      cx<T>* __restrict__ carry = J.comp[field];
+      if (carry) {
+        const cx<T> lo = carry[idx];
+        const T yx = ix - lo.x, yy = iy - lo.y;
+        const T tx = v.x + yx, ty = v.y + yy;
+        cx<T> nlo = mk<T>((tx - v.x) - yx, (ty - v.y) - yy);
+        if (J.problem == MLV_PROB_DDC && field != 0 && m == 0) nlo = czero<T>();
+        carry[idx] = nlo;
+        nv = mk<T>(tx, ty);
+      } else {
+        nv = mk<T>(v.x + ix, v.y + iy);
+      }
+    }
     if (J.problem == MLV_PROB_DDC && field != 0 && m == 0) nv = czero<T>();
     sout[idx] = nv;
   }
@@ -332,16 +366,14 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
 template <typename T>
 __global__ void __launch_bounds__(kNT) k_stepC(const JobDev<T>* __restrict__ jobs,
                                                const int* __restrict__ pref, int njobs, int stage,
-                                               const long long* __restrict__ step,
-                                               const cx<T>* __restrict__ tw) {
+                                               int step, const cx<T>* __restrict__ tw) {
   extern __shared__ char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int j = find_job(pref, njobs, blockIdx.x);
   const JobDev<T>& J = jobs[j];
   const int local = blockIdx.x - pref[j];
-  const long long st = *step;
-  const int par = (int)((J.par0 + st) & 1);
-  const int cur = ring(J.curr0, st, J.q);
+  const int par = (J.par0 + step) & 1;
+  const int cur = ring(J.curr0, step, J.q);
   switch (J.lognx) {
 #define X(L)                                                            \
   case L:                                                               \
@@ -352,10 +384,6 @@ __global__ void __launch_bounds__(kNT) k_stepC(const JobDev<T>* __restrict__ job
     default:
       break;
   }
-}
-
-__global__ void k_advance(long long* step) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
 }
 
 template <typename T>
@@ -387,30 +415,47 @@ static void launchB(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int
 
 template <typename T>
 hipError_t fused_stage(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int wphys,
-                       const long long* step, const cx<T>* tw, hipStream_t s) {
+                       int step, const cx<T>* tw, hipStream_t s, hipEvent_t* ev) {
+  if (ev) (void)hipEventRecord(ev[0], s);
   if (bl.nA[stage] > 0) {
     MLV_SET_LDS_ONCE(k_stepA<T>, bl.ldsA);
     hipLaunchKernelGGL(k_stepA<T>, dim3(bl.nA[stage]), dim3(kNT), bl.ldsA, s, jobs,
                        (const int*)bl.prefA[stage], bl.njobs, stage, step, tw);
   }
+  if (ev) (void)hipEventRecord(ev[1], s);
   launchB<T, BV_VORT1>(jobs, bl, stage, wphys, tw, s);
   launchB<T, BV_VORT2>(jobs, bl, stage, wphys, tw, s);
   launchB<T, BV_VORT3>(jobs, bl, stage, wphys, tw, s);
   launchB<T, BV_MHD1>(jobs, bl, stage, wphys, tw, s);
   launchB<T, BV_MHD2>(jobs, bl, stage, wphys, tw, s);
+  if (ev) (void)hipEventRecord(ev[2], s);
   if (bl.nC[stage] > 0) {
     MLV_SET_LDS_ONCE(k_stepC<T>, bl.ldsC);
     hipLaunchKernelGGL(k_stepC<T>, dim3(bl.nC[stage]), dim3(kNT), bl.ldsC, s, jobs,
                        (const int*)bl.prefC[stage], bl.njobs, stage, step, tw);
   }
+  if (ev) (void)hipEventRecord(ev[3], s);
   return hipGetLastError();
 }
 
+// Raise the dynamic-LDS limit of every fused kernel once, outside any
+// stream capture.
 template <typename T>
-hipError_t fused_advance(JobDev<T>*, int, long long* step, double*, hipStream_t s) {
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, step);
+hipError_t fused_prepare(const BatchLaunch& bl) {
+  size_t b = 0;
+  for (int v = 0; v < BV_COUNT; ++v) b = bl.ldsB[v] > b ? bl.ldsB[v] : b;
+  if (bl.ldsA) MLV_SET_LDS_ONCE(k_stepA<T>, bl.ldsA);
+  if (bl.ldsC) MLV_SET_LDS_ONCE(k_stepC<T>, bl.ldsC);
+  if (bl.ldsB[BV_VORT1]) MLV_SET_LDS_ONCE((k_stepB<T, BV_VORT1>), bl.ldsB[BV_VORT1]);
+  if (bl.ldsB[BV_VORT2]) MLV_SET_LDS_ONCE((k_stepB<T, BV_VORT2>), bl.ldsB[BV_VORT2]);
+  if (bl.ldsB[BV_VORT3]) MLV_SET_LDS_ONCE((k_stepB<T, BV_VORT3>), bl.ldsB[BV_VORT3]);
+  if (bl.ldsB[BV_MHD1]) MLV_SET_LDS_ONCE((k_stepB<T, BV_MHD1>), bl.ldsB[BV_MHD1]);
+  if (bl.ldsB[BV_MHD2]) MLV_SET_LDS_ONCE((k_stepB<T, BV_MHD2>), bl.ldsB[BV_MHD2]);
+  (void)b;
   return hipGetLastError();
 }
+template hipError_t fused_prepare<float>(const BatchLaunch&);
+template hipError_t fused_prepare<double>(const BatchLaunch&);
 
 template <typename T>
 hipError_t fused_set_dt(JobDev<T>* jobs, int job, double dt, hipStream_t s) {
@@ -420,11 +465,9 @@ hipError_t fused_set_dt(JobDev<T>* jobs, int job, double dt, hipStream_t s) {
 
 template hipError_t fused_set_dt<float>(JobDev<float>*, int, double, hipStream_t);
 template hipError_t fused_set_dt<double>(JobDev<double>*, int, double, hipStream_t);
-template hipError_t fused_stage<float>(const JobDev<float>*, const BatchLaunch&, int, int,
-                                       const long long*, const cx<float>*, hipStream_t);
-template hipError_t fused_stage<double>(const JobDev<double>*, const BatchLaunch&, int, int,
-                                        const long long*, const cx<double>*, hipStream_t);
-template hipError_t fused_advance<float>(JobDev<float>*, int, long long*, double*, hipStream_t);
-template hipError_t fused_advance<double>(JobDev<double>*, int, long long*, double*, hipStream_t);
+template hipError_t fused_stage<float>(const JobDev<float>*, const BatchLaunch&, int, int, int,
+                                       const cx<float>*, hipStream_t, hipEvent_t*);
+template hipError_t fused_stage<double>(const JobDev<double>*, const BatchLaunch&, int, int, int,
+                                        const cx<double>*, hipStream_t, hipEvent_t*);
 
 }  // namespace mlv

@@ -70,6 +70,7 @@ struct JobDev {
   int nfield;
   cx<T>* state[2][kMaxFields];
   cx<T>* hist[kMaxFields];
+  cx<T>* comp[kMaxFields];   // optional Kahan carry of the state (single precision)
   cx<T>* work;
   cx<T>* psi;
   T* phys[kMaxPhys];
@@ -143,12 +144,15 @@ struct BatchLaunch {
   int* prefC[2];
 };
 
+// `step` is the batch's step count modulo 4 (a multiple of both ring
+// lengths and of the ping-pong period), so a captured sequence of 4k steps
+// replays correctly.
 template <typename T>
 hipError_t fused_stage(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int wphys,
-                       const long long* step, const cx<T>* tw, hipStream_t s);
-template <typename T>
-hipError_t fused_advance(JobDev<T>* jobs, int njobs, long long* step, double* t, hipStream_t s);
+                       int step, const cx<T>* tw, hipStream_t s, hipEvent_t* ev = nullptr);
 template <typename T>
 hipError_t fused_set_dt(JobDev<T>* jobs, int job, double dt, hipStream_t s);
+template <typename T>
+hipError_t fused_prepare(const BatchLaunch& bl);
 
 }  // namespace mlv

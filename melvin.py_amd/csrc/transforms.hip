@@ -26,7 +26,7 @@ __global__ void __launch_bounds__(kNT) k_xinv(const cx<T>* __restrict__ in,
                                               cx<T>* __restrict__ out, int nn, int nm,
                                               const cx<T>* __restrict__ tw) {
   constexpr int N = 1 << LOGN;
-  constexpr int P = fft_points(LOGN);
+  constexpr int P = fft_points_col<T>(LOGN);
   constexpr int TC = fft_slots<LOGN, kNT, P>();
   constexpr int LDSN = fft_ldsn(N);
   extern __shared__ char smem_raw[];
@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(kNT) k_xfwd(const cx<T>* __restrict__ in,
                                               cx<T>* __restrict__ out, int nn, int nm,
                                               T scale, const cx<T>* __restrict__ tw) {
   constexpr int N = 1 << LOGN;
-  constexpr int P = fft_points(LOGN);
+  constexpr int P = fft_points_col<T>(LOGN);
   constexpr int TC = fft_slots<LOGN, kNT, P>();
   constexpr int LDSN = fft_ldsn(N);
   extern __shared__ char smem_raw[];
@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(kNT) k_zc2r(const cx<T>* __restrict__ in,
                                               T* __restrict__ out, int nx, int nm,
                                               const cx<T>* __restrict__ tw) {
   constexpr int N = 1 << LOGN;  // nz
-  constexpr int P = fft_points(LOGN);
+  constexpr int P = fft_points_row<T>(LOGN);
   constexpr int S = fft_slots<LOGN, kNT, P>();
   constexpr int LDSN = fft_ldsn(N);
   extern __shared__ char smem_raw[];
@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(kNT) k_zr2c(const T* __restrict__ in,
                                               cx<T>* __restrict__ out, int nx, int nm,
                                               const cx<T>* __restrict__ tw) {
   constexpr int N = 1 << LOGN;
-  constexpr int P = fft_points(LOGN);
+  constexpr int P = fft_points_row<T>(LOGN);
   constexpr int S = fft_slots<LOGN, kNT, P>();
   constexpr int LDSN = fft_ldsn(N);
   extern __shared__ char smem_raw[];
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(kNT) k_xc2r_cols(const cx<T>* __restrict__ in,
                                                    T* __restrict__ out, int nn, int nz,
                                                    const cx<T>* __restrict__ tw) {
   constexpr int N = 1 << LOGN;  // nx
-  constexpr int P = fft_points(LOGN);
+  constexpr int P = fft_points_col<T>(LOGN);
   constexpr int S = fft_slots<LOGN, kNT, P>();
   constexpr int LDSN = fft_ldsn(N);
   extern __shared__ char smem_raw[];
@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(kNT) k_xr2c_cols(const T* __restrict__ in,
                                                    cx<T>* __restrict__ out, int nn, int nz,
                                                    T scale, const cx<T>* __restrict__ tw) {
   constexpr int N = 1 << LOGN;
-  constexpr int P = fft_points(LOGN);
+  constexpr int P = fft_points_col<T>(LOGN);
   constexpr int S = fft_slots<LOGN, kNT, P>();
   constexpr int LDSN = fft_ldsn(N);
   extern __shared__ char smem_raw[];
@@ -307,9 +307,9 @@ hipError_t launch_xfwd(int logn, const void* in, void* out, int nn, int nm, doub
 template <typename T>
 hipError_t launch_zc2r(int logn, const void* in, void* out, int nx, int nm, const void* tw,
                        hipStream_t s) {
-  const int rows = 2 * slots_for<T>(logn);
+  const int rows = 2 * slots_row<T>(logn);
   const dim3 grid((nx + rows - 1) / rows);
-  const size_t lds = fft_lds_bytes<T>(logn);
+  const size_t lds = fft_lds_bytes_row<T>(logn);
   switch (logn) {
 #define X(L)                                                                               \
   case L:                                                                                  \
@@ -328,9 +328,9 @@ hipError_t launch_zc2r(int logn, const void* in, void* out, int nx, int nm, cons
 template <typename T>
 hipError_t launch_zr2c(int logn, const void* in, void* out, int nx, int nm, const void* tw,
                        hipStream_t s) {
-  const int rows = 2 * slots_for<T>(logn);
+  const int rows = 2 * slots_row<T>(logn);
   const dim3 grid((nx + rows - 1) / rows);
-  const size_t lds = fft_lds_bytes<T>(logn);
+  const size_t lds = fft_lds_bytes_row<T>(logn);
   switch (logn) {
 #define X(L)                                                                               \
   case L:                                                                                  \

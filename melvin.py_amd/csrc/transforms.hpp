@@ -8,15 +8,23 @@ namespace mlv {
 // Threads per block of every FFT-carrying kernel.
 constexpr int kNT = 256;
 
-// LDS bytes of one block transforming slots_for(logn) sequences of 2^logn.
-template <typename T>
-inline size_t fft_lds_bytes(int logn) {
-  const int slots = (kNT * fft_points(logn)) >> logn;
-  return (size_t)slots * fft_ldsn(1 << logn) * sizeof(cx<T>);
-}
+// Sequences of 2^logn one block of a column (x) pass transforms, and the
+// LDS they take; the _row versions are for the row (z) passes.
 template <typename T>
 inline int slots_for(int logn) {
-  return (kNT * fft_points(logn)) >> logn;
+  return (kNT * fft_points_col<T>(logn)) >> logn;
+}
+template <typename T>
+inline size_t fft_lds_bytes(int logn) {
+  return (size_t)slots_for<T>(logn) * fft_ldsn(1 << logn) * sizeof(cx<T>);
+}
+template <typename T>
+inline int slots_row(int logn) {
+  return (kNT * fft_points_row<T>(logn)) >> logn;
+}
+template <typename T>
+inline size_t fft_lds_bytes_row(int logn) {
+  return (size_t)slots_row<T>(logn) * fft_ldsn(1 << logn) * sizeof(cx<T>);
 }
 
 template <typename T>

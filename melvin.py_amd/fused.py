@@ -47,7 +47,7 @@ class Job:
 
     def __init__(self, problem, nx, nz, lx, lz, dtype="f32", dt=1e-3, coeff=(0.0,),
                  param=(), nfield=None, ab_order=2, fd_order=2, alpha=0.51,
-                 want_psi=False, want_phys=False, device=None):
+                 want_psi=False, want_phys=False, compensated=True, device=None):
         if problem not in PROBLEMS:
             raise ValueError(f"unknown problem {problem!r}")
         self.problem = problem
@@ -77,6 +77,10 @@ class Job:
                     for _ in range(2)]
         self.hist = [torch.zeros((self.ab_order,) + shape, dtype=self.cplx, device=dev)
                      for _ in range(nf)]
+        # single precision: Kahan carry of the state update (see the x-forward
+        # kernel), so the stored state is buf + carry
+        self.carry = ([torch.zeros(shape, dtype=self.cplx, device=dev) for _ in range(nf)]
+                      if (compensated and not self.double) else None)
         nbytes = ctypes.c_size_t()
         _lib.call("mlv_job_work_bytes", PROBLEMS[problem], nf, self.nx, self.nz,
                   _lib.F64 if self.double else _lib.F32, ctypes.byref(nbytes))
@@ -97,6 +101,8 @@ class Job:
     def hbm_bytes(self):
         n = sum(b.numel() * b.element_size() for row in self.buf for b in row)
         n += sum(h.numel() * h.element_size() for h in self.hist)
+        if self.carry is not None:
+            n += sum(c.numel() * c.element_size() for c in self.carry)
         n += self.work.numel()
         if self.psi is not None:
             n += self.psi.numel() * self.psi.element_size()
@@ -113,15 +119,26 @@ class Job:
             if tuple(t.shape) != self.spectral_shape:
                 raise ValueError(f"field {i} has shape {tuple(t.shape)}, expected "
                                  f"{self.spectral_shape}")
-            self.buf[self.parity][i].copy_(t.to(self.buf[0][0].device, self.cplx))
+            t = t.to(self.buf[0][0].device, torch.complex128)
+            hi = t.to(self.cplx)
+            self.buf[self.parity][i].copy_(hi)
+            if self.carry is not None:
+                self.carry[i].copy_((t - hi.to(torch.complex128)).to(self.cplx))
 
     def current(self):
         """Device tensors of the current spectral state."""
         return self.buf[self.parity]
 
     def state(self):
-        """Current spectral state as complex128 host arrays."""
-        return [b.detach().cpu().numpy().astype(np.complex128) for b in self.current()]
+        """Current spectral state as complex128 host arrays (including the
+        single-precision carry when there is one)."""
+        out = []
+        for i, b in enumerate(self.current()):
+            v = b.detach().cpu().numpy().astype(np.complex128)
+            if self.carry is not None:
+                v = v + self.carry[i].detach().cpu().numpy().astype(np.complex128)
+            out.append(v)
+        return out
 
     def physical(self):
         """Physical fields written by the last step run with write_phys."""
@@ -143,6 +160,8 @@ class Job:
                 d.state[b][f] = self.buf[b][f].data_ptr()
         for f in range(self.nfield):
             d.hist[f] = self.hist[f].data_ptr()
+            if self.carry is not None:
+                d.comp[f] = self.carry[f].data_ptr()
         d.work = self.work.data_ptr()
         d.psi = self.psi.data_ptr() if self.psi is not None else None
         if self.phys is not None:
@@ -168,6 +187,7 @@ class Batch:
         _lib.call("mlv_batch_create", ctypes.byref(h), _lib.F64 if self.double else _lib.F32,
                   arr, len(self.jobs))
         self.handle = h
+        self._steps0 = [j.steps for j in self.jobs]
 
     def step(self, nsteps=1, write_phys=False, stream=None):
         """Queue ``nsteps`` steps of every job on the stream (asynchronous)."""
@@ -175,6 +195,20 @@ class Batch:
             return
         _lib.call("mlv_batch_step", self.handle, int(nsteps), 1 if write_phys else 0,
                   _lib.stream_handle(stream))
+        for j in self.jobs:
+            j.steps += nsteps
+            j.curr = (j.curr + nsteps) % j.ab_order
+            j.parity = (j.parity + nsteps) & 1
+            j.t += nsteps * j.dt
+
+    def graph_step(self, nsteps, stream=None):
+        """Queue ``nsteps`` (a multiple of 4) steps as one HIP graph launch;
+        the graph is captured on first use and replayed afterwards."""
+        if nsteps % 4:
+            raise ValueError("graph steps must be a multiple of 4")
+        if (self.jobs[0].steps - self._steps0[0]) % 4:
+            raise ValueError("graph replay needs the batch step count at a multiple of 4")
+        _lib.call("mlv_batch_graph_step", self.handle, int(nsteps), _lib.stream_handle(stream))
         for j in self.jobs:
             j.steps += nsteps
             j.curr = (j.curr + nsteps) % j.ab_order
