@@ -103,6 +103,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (a known-size kernel "
+                         "for scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N > 1 is not "
@@ -119,6 +122,13 @@ def main():
     m = __graft_entry__.load_package()
     fused = m.fused
     torch.cuda.init()
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     print(f"[bench] device {torch.cuda.get_device_name(0)}; batch {args.batch} x "
           f"{NX}x{NZ} FP32; {args.warmup} warm-up + {args.steps} timed steps",
           file=sys.stderr, flush=True)

@@ -13,7 +13,7 @@ The directory is ``melvin.py_amd``; it is imported under the package name
 ``melvin_amd`` (see ``__graft_entry__.load_package``).
 """
 from . import hip
-from . import fused, utility
+from . import fused, initial, serving, utility
 from .ArrayFactory import ArrayFactory
 from .BasisFunctions import BasisFunctions
 from .DataTransferer import DataTransferer
@@ -30,7 +30,7 @@ from .Variable import Variable
 from .Simulation import Simulation
 
 __all__ = [
-    "hip", "fused", "utility", "ArrayFactory", "BasisFunctions", "DataTransferer", "Integrator",
+    "hip", "fused", "initial", "serving", "utility", "ArrayFactory", "BasisFunctions", "DataTransferer", "Integrator",
     "LaplacianSolver", "Parameters", "ScalarTracker", "SpatialDifferentiator",
     "SpectralTransformer", "Ticker", "TimeDerivative", "Timer", "Variable",
     "Simulation",

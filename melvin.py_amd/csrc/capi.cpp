@@ -725,7 +725,7 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
         bvar[s] = J.st[s].bvar;
       }
     }
-    const int slx = dtype == MLV_F64 ? slots_for<double>(lognx) : slots_for<float>(lognx);
+    const int slx = dtype == MLV_F64 ? fused_cols_host<double>(lognx) : fused_cols_host<float>(lognx);
     const int slz = dtype == MLV_F64 ? slots_row<double>(lognz) : slots_row<float>(lognz);
     const size_t ldsx = dtype == MLV_F64 ? fft_lds_bytes<double>(lognx) : fft_lds_bytes<float>(lognx);
     const size_t ldsz =

@@ -17,6 +17,19 @@ __device__ __forceinline__ int find_job(const int* __restrict__ pref, int njobs,
 
 __device__ __forceinline__ int ring(int c0, int step, int q) { return (c0 + step) % q; }
 
+// Workgroups of a launch are dealt to the 8 XCDs round robin (block b runs
+// on XCD b % 8).  Give each XCD a contiguous range of logical blocks, so
+// neighbouring column tiles -- which share 128-byte lines whenever a tile is
+// narrower than a line -- are fetched through the same L2.  The grid is
+// padded to a multiple of 8; returns -1 for padding blocks.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_logical(int nlogical) {
+  const int per = (nlogical + kXcds - 1) / kXcds;
+  const int b = blockIdx.x;
+  const int l = (b % kXcds) * per + b / kXcds;
+  return l < nlogical ? l : -1;
+}
+
 // --------------------------------------------------------------------------
 // Kernel A: spectral prologue + inverse x pass
 // --------------------------------------------------------------------------
@@ -25,7 +38,7 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
                                            int par, const cx<T>* __restrict__ tw, cx<T>* lds) {
   constexpr int N = 1 << LOGN;
   constexpr int P = fft_points_col<T>(LOGN);
-  constexpr int TC = fft_slots<LOGN, kNT, P>();
+  constexpr int TC = fused_cols<LOGN, P>();
   constexpr int LDSN = fft_ldsn(N);
   const int nn = J.nn, nm = J.nm, rows = J.rows;
   const int ntile = (nm + TC - 1) / TC;
@@ -37,7 +50,24 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
   cx<T>* __restrict__ dst = J.work + (size_t)op.ch * N * nm;
   cx<T>* __restrict__ psi_out = (op.op == XOP_UX && op.src == 0 && !op.from_new) ? J.psi : nullptr;
   const double fx = J.fx, fz = J.fz, fx2 = J.fx2, fz2 = J.fz2;
-  for (int e = threadIdx.x; e < N * TC; e += kNT) {
+  // Each thread owns ITER = P elements; loads are issued U at a time so
+  // several are in flight before the first is used.
+  constexpr int ITER = (N * TC) / kNT;
+  constexpr int U = 8;
+  static_assert(ITER % U == 0, "load batching");
+#pragma unroll 1
+  for (int i0 = 0; i0 < ITER; i0 += U) {
+    cx<T> fv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = threadIdx.x + (i0 + u) * kNT;
+      const int c = e % TC, x = e / TC;
+      const int r = x <= nn ? x : (x >= N - nn ? x - N + rows : -1);
+      fv[u] = (c < ncol && r >= 0) ? src[(size_t)r * nm + c0 + c] : czero<T>();
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+    const int e = threadIdx.x + (i0 + u) * kNT;
     const int c = e % TC, x = e / TC;
     cx<T> v = czero<T>();
     if (c < ncol) {
@@ -45,7 +75,7 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
       if (r >= 0) {
         const int m = c0 + c;
         const size_t idx = (size_t)r * nm + m;
-        const cx<T> f = src[idx];
+        const cx<T> f = fv[u];
         if (op.op == XOP_ID) {
           v = f;
         } else {
@@ -67,10 +97,13 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
       }
     }
     lds[c * LDSN + fft_pad(x)] = v;
+    }
   }
   __syncthreads();
   fft_lds<T, LOGN, +1, kNT, P>(lds, ncol, tw);
-  for (int e = threadIdx.x; e < N * TC; e += kNT) {
+#pragma unroll 8
+  for (int i = 0; i < ITER; ++i) {
+    const int e = threadIdx.x + i * kNT;
     const int c = e % TC, x = e / TC;
     if (c < ncol) dst[(size_t)x * nm + c0 + c] = lds[c * LDSN + fft_pad(x)];
   }
@@ -82,9 +115,11 @@ __global__ void __launch_bounds__(kNT) k_stepA(const JobDev<T>* __restrict__ job
                                                int step, const cx<T>* __restrict__ tw) {
   extern __shared__ char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
-  const int j = find_job(pref, njobs, blockIdx.x);
+  const int lb = xcd_logical(pref[njobs]);
+  if (lb < 0) return;
+  const int j = find_job(pref, njobs, lb);
   const JobDev<T>& J = jobs[j];
-  const int local = blockIdx.x - pref[j];
+  const int local = lb - pref[j];
   const int par = (J.par0 + step) & 1;
   switch (J.lognx) {
 #define X(L)                                                       \
@@ -126,19 +161,35 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
       const int s = e / nzero, k = nm + e % nzero;
       lds[s * LDSN + fft_pad(k)] = czero<T>();
     }
-    for (int e = threadIdx.x; e < nslot * nm; e += kNT) {
-      const int s = e / nm, k = e % nm;
-      const int qa = 2 * s, qb = qa + 1;
-      const cx<T> a = work[(size_t)(qa % NIN) * chs + (size_t)(x0 + qa / NIN) * nm + k];
-      const cx<T> b = qb < nseq
-                          ? work[(size_t)(qb % NIN) * chs + (size_t)(x0 + qb / NIN) * nm + k]
+    constexpr int UB = 4;
+    const int nfill = nslot * nm;
+#pragma unroll 1
+    for (int e0 = threadIdx.x; e0 < nfill; e0 += UB * kNT) {
+      cx<T> av[UB], bv[UB];
+#pragma unroll
+      for (int u = 0; u < UB; ++u) {
+        const int e = e0 + u * kNT;
+        const int s = e / nm, k = e % nm;
+        const int qa = 2 * s, qb = qa + 1;
+        av[u] = e < nfill ? work[(size_t)(qa % NIN) * chs + (size_t)(x0 + qa / NIN) * nm + k]
                           : czero<T>();
-      cx<T>* z = lds + s * LDSN;
-      if (k == 0) {
-        z[0] = mk<T>(a.x, b.x);
-      } else {
-        z[fft_pad(k)] = mk<T>(a.x - b.y, a.y + b.x);
-        z[fft_pad(N - k)] = mk<T>(a.x + b.y, b.x - a.y);
+        bv[u] = (e < nfill && qb < nseq)
+                    ? work[(size_t)(qb % NIN) * chs + (size_t)(x0 + qb / NIN) * nm + k]
+                    : czero<T>();
+      }
+#pragma unroll
+      for (int u = 0; u < UB; ++u) {
+        const int e = e0 + u * kNT;
+        if (e >= nfill) continue;
+        const int s = e / nm, k = e % nm;
+        const cx<T> a = av[u], b = bv[u];
+        cx<T>* z = lds + s * LDSN;
+        if (k == 0) {
+          z[0] = mk<T>(a.x, b.x);
+        } else {
+          z[fft_pad(k)] = mk<T>(a.x - b.y, a.y + b.x);
+          z[fft_pad(N - k)] = mk<T>(a.x + b.y, b.x - a.y);
+        }
       }
     }
     __syncthreads();
@@ -195,9 +246,11 @@ __global__ void __launch_bounds__(kNT) k_stepB(const JobDev<T>* __restrict__ job
                                                int wphys, const cx<T>* __restrict__ tw) {
   extern __shared__ char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
-  const int j = find_job(pref, njobs, blockIdx.x);
+  const int lb = xcd_logical(pref[njobs]);
+  if (lb < 0) return;
+  const int j = find_job(pref, njobs, lb);
   const JobDev<T>& J = jobs[j];
-  const int local = blockIdx.x - pref[j];
+  const int local = lb - pref[j];
   switch (J.lognz) {
 #define X(L)                                                      \
   case L:                                                         \
@@ -227,7 +280,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
                                            int par, const cx<T>* __restrict__ tw, cx<T>* lds) {
   constexpr int N = 1 << LOGN;
   constexpr int P = fft_points_col<T>(LOGN);
-  constexpr int TC = fft_slots<LOGN, kNT, P>();
+  constexpr int TC = fused_cols<LOGN, P>();
   constexpr int LDSN = fft_ldsn(N);
   const int nn = J.nn, nm = J.nm, rows = J.rows;
   const size_t chs = (size_t)N * nm;
@@ -245,26 +298,47 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   //   D = F_x[ FDx(Px) + i sz(kz) Pz ].
   const int order = J.fd_order;
   const T ihx = order == 2 ? (T)(1.0 / (2.0 * J.dx)) : (T)(1.0 / (3.0 * J.dx));
-  for (int e = threadIdx.x; e < N * TC; e += kNT) {
-    const int c = e % TC, x = e / TC;
-    cx<T> v = czero<T>();
-    if (c < ncol) {
-      const int m = c0 + c;
-      auto px = [&](int dxr) { return pxc[(size_t)((x + dxr) & (N - 1)) * nm + m]; };
-      cx<T> d;
-      if (order == 2) {
-        const cx<T> a = px(1), b = px(-1);
-        d = mk<T>((a.x - b.x) * ihx, (a.y - b.y) * ihx);
-      } else {
-        const cx<T> a2 = px(2), a1 = px(1), b1 = px(-1), b2 = px(-2);
-        d = mk<T>((T(-0.25) * a2.x + T(2) * a1.x - T(2) * b1.x + T(0.25) * b2.x) * ihx,
-                  (T(-0.25) * a2.y + T(2) * a1.y - T(2) * b1.y + T(0.25) * b2.y) * ihx);
+  constexpr int ITER = (N * TC) / kNT;
+  constexpr int U = 8;
+  static_assert(ITER % U == 0, "load batching");
+  // sz depends only on the column; every thread keeps one column (e % TC
+  // is fixed because kNT is a multiple of TC)
+  const int cme = threadIdx.x % TC;
+  const T szc = modk<T>(tw, c0 + cme, J.lognz, order, J.dz);
+  const bool cok = cme < ncol;
+#pragma unroll 1
+  for (int i0 = 0; i0 < ITER; i0 += U) {
+    cx<T> pa[U], pb[U], pz[U], pa2[U], pb2[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int x = (threadIdx.x + (i0 + u) * kNT) / TC;
+      const size_t col = (size_t)(c0 + cme);
+      if (cok) {
+        pa[u] = pxc[(size_t)((x + 1) & (N - 1)) * nm + col];
+        pb[u] = pxc[(size_t)((x - 1) & (N - 1)) * nm + col];
+        pz[u] = pzc[(size_t)x * nm + col];
+        if (order != 2) {
+          pa2[u] = pxc[(size_t)((x + 2) & (N - 1)) * nm + col];
+          pb2[u] = pxc[(size_t)((x - 2) & (N - 1)) * nm + col];
+        }
       }
-      const T sz = modk<T>(tw, m, J.lognz, order, J.dz);
-      const cx<T> z = pzc[(size_t)x * nm + m];
-      v = mk<T>(d.x - sz * z.y, d.y + sz * z.x);
     }
-    lds[c * LDSN + fft_pad(x)] = v;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int x = (threadIdx.x + (i0 + u) * kNT) / TC;
+      cx<T> v = czero<T>();
+      if (cok) {
+        cx<T> d;
+        if (order == 2) {
+          d = mk<T>((pa[u].x - pb[u].x) * ihx, (pa[u].y - pb[u].y) * ihx);
+        } else {
+          d = mk<T>((T(-0.25) * pa2[u].x + T(2) * pa[u].x - T(2) * pb[u].x + T(0.25) * pb2[u].x) * ihx,
+                    (T(-0.25) * pa2[u].y + T(2) * pa[u].y - T(2) * pb[u].y + T(0.25) * pb2[u].y) * ihx);
+        }
+        v = mk<T>(d.x - szc * pz[u].y, d.y + szc * pz[u].x);
+      }
+      lds[cme * LDSN + fft_pad(x)] = v;
+    }
   }
   __syncthreads();
   fft_lds<T, LOGN, -1, kNT, P>(lds, ncol, tw);
@@ -278,88 +352,112 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   const cx<T>* __restrict__ sin_ = J.state[par][field];
   cx<T>* __restrict__ sout = J.state[par ^ 1][field];
   cx<T>* __restrict__ hist = J.hist[field];
-  for (int e = threadIdx.x; e < rows * TC; e += kNT) {
-    const int c = e % TC, r = e / TC;
-    if (c >= ncol) continue;
-    const int x = r <= nn ? r : r + N - rows;
-    const int n = r <= nn ? r : r - rows;
-    const int m = c0 + c;
-    const size_t idx = (size_t)r * nm + m;
-    const cx<T> D = lds[c * LDSN + fft_pad(x)];
-    cx<T> rhs = mk<T>(D.x * scale, D.y * scale);
-    if (J.problem == MLV_PROB_DDC) {
-      const double Pr = J.param[0], R0 = J.param[1];
-      if (field == 0) {
-        const T y = (T)(J.fx * n);
-        const cx<T> s = J.state[par][2][idx], t = J.state[par][1][idx];
-        const T pr = (T)Pr;
-        rhs = cadd<T>(rhs, mk<T>(pr * -(y * s.y), pr * (y * s.x)));
-        rhs = csub<T>(rhs, mk<T>(pr * -(y * t.y), pr * (y * t.x)));
-      } else {
-        double lap = J.fx2 * (double)(n * n) + J.fz2 * (double)(m * m);
-        if (r == 0 && m == 0) lap = 1.0;
-        const T rl = (T)(1.0 / lap);
-        const cx<T> w = J.state[par][0][idx];
-        const cx<T> ps = mk<T>(-w.x * rl, -w.y * rl);
-        const T y = (T)(J.fx * n);
-        cx<T> uz = mk<T>(-(y * ps.y), y * ps.x);
-        if (field == 2) {
-          const T ir0 = (T)(1.0 / R0);
-          uz = mk<T>(uz.x * ir0, uz.y * ir0);
+  // Epilogue: every thread keeps column cme; its elements are rows
+  // r = threadIdx.x / TC + i * (kNT / TC).  Loads of U2 elements are issued
+  // before any is used.
+  constexpr int RSTEP = kNT / TC;
+  constexpr int U2 = 4;
+  const int m = c0 + cme;
+  const int r0 = threadIdx.x / TC;
+  const bool ddc = J.problem == MLV_PROB_DDC;
+  cx<T>* __restrict__ carry = sizeof(T) == 4 ? J.comp[field] : nullptr;
+  const int prev1 = (cur + q - 1) % q, prev2 = (cur + q - 2) % q, prev3 = (cur + q - 3) % q;
+#pragma unroll 1
+  for (int rb = r0; rb < rows; rb += U2 * RSTEP) {
+    cx<T> vv[U2], h1[U2], h2[U2], h3[U2], lo[U2], e1[U2], e2[U2];
+#pragma unroll
+    for (int u = 0; u < U2; ++u) {
+      const int r = rb + u * RSTEP;
+      if (cok && r < rows) {
+        const size_t idx = (size_t)r * nm + m;
+        vv[u] = sin_[idx];
+        h1[u] = hist[(size_t)prev1 * ssz + idx];
+        if (q == 4) {
+          h2[u] = hist[(size_t)prev2 * ssz + idx];
+          h3[u] = hist[(size_t)prev3 * ssz + idx];
         }
-        rhs = csub<T>(rhs, uz);
+        if (carry) lo[u] = carry[idx];
+        if (ddc) {
+          if (field == 0) {
+            e1[u] = J.state[par][1][idx];
+            e2[u] = J.state[par][2][idx];
+          } else {
+            e1[u] = J.state[par][0][idx];
+          }
+        }
       }
     }
-    hist[(size_t)cur * ssz + idx] = rhs;
-    auto hget = [&](int i) {
-      int s = cur + i;
-      if (s < 0) s += q;
-      return hist[(size_t)s * ssz + idx];
-    };
-    T abx, aby;
-    {
-      const cx<T> d1 = hget(-1);
+#pragma unroll
+    for (int u = 0; u < U2; ++u) {
+      const int r = rb + u * RSTEP;
+      if (!cok || r >= rows) continue;
+      const int x = r <= nn ? r : r + N - rows;
+      const int n = r <= nn ? r : r - rows;
+      const size_t idx = (size_t)r * nm + m;
+      const cx<T> D = lds[cme * LDSN + fft_pad(x)];
+      cx<T> rhs = mk<T>(D.x * scale, D.y * scale);
+      if (ddc) {
+        const double Pr = J.param[0], R0 = J.param[1];
+        if (field == 0) {
+          const T y = (T)(J.fx * n);
+          const cx<T> t = e1[u], sv = e2[u];
+          const T pr = (T)Pr;
+          rhs = cadd<T>(rhs, mk<T>(pr * -(y * sv.y), pr * (y * sv.x)));
+          rhs = csub<T>(rhs, mk<T>(pr * -(y * t.y), pr * (y * t.x)));
+        } else {
+          double lap = J.fx2 * (double)(n * n) + J.fz2 * (double)(m * m);
+          if (r == 0 && m == 0) lap = 1.0;
+          const T rl = (T)(1.0 / lap);
+          const cx<T> w = e1[u];
+          const cx<T> ps = mk<T>(-w.x * rl, -w.y * rl);
+          const T y = (T)(J.fx * n);
+          cx<T> uz = mk<T>(-(y * ps.y), y * ps.x);
+          if (field == 2) {
+            const T ir0 = (T)(1.0 / R0);
+            uz = mk<T>(uz.x * ir0, uz.y * ir0);
+          }
+          rhs = csub<T>(rhs, uz);
+        }
+      }
+      hist[(size_t)cur * ssz + idx] = rhs;
+      T abx, aby;
       if (q == 2) {
-        abx = T(3) * rhs.x - d1.x;
-        aby = T(3) * rhs.y - d1.y;
+        abx = T(3) * rhs.x - h1[u].x;
+        aby = T(3) * rhs.y - h1[u].y;
       } else {
-        const cx<T> d2 = hget(-2), d3 = hget(-3);
-        abx = T(55) * rhs.x - T(59) * d1.x + T(37) * d2.x - T(9) * d3.x;
-        aby = T(55) * rhs.y - T(59) * d1.y + T(37) * d2.y - T(9) * d3.y;
+        abx = T(55) * rhs.x - T(59) * h1[u].x + T(37) * h2[u].x - T(9) * h3[u].x;
+        aby = T(55) * rhs.y - T(59) * h1[u].y + T(37) * h2[u].y - T(9) * h3[u].y;
       }
-    }
-    const double lin = coeff * (J.fx2 * (double)(n * n) + J.fz2 * (double)(m * m));
-    const double den = 1.0 - c2 * lin;
-    const cx<T> v = sin_[idx];
-    cx<T> nv;
-    if constexpr (sizeof(T) == 8) {
-      // (1 + (1-a) dt L) v + AB, divided by (1 - a dt L)  (Integrator.py:58-63)
-      const T num = (T)(1.0 + c1 * lin);
-      const T rden = (T)(1.0 / den);
-      nv = mk<T>((num * v.x + (T)hab * abx) * rden, (num * v.y + (T)hab * aby) * rden);
-    } else {
-      // The same update written as an increment, v + (dt L v + AB)/(1 - a dt L),
-      // so the rounding falls on the small change, not on (1 + (1-a) dt L).
-      // With a carry array the addition is Kahan-compensated: increments
-      // below half an ulp of v are kept instead of being dropped every step.
-      const T a = (T)(dt * lin);
-      const T rden = (T)(1.0 / den);
-      const T ix = (a * v.x + (T)hab * abx) * rden, iy = (a * v.y + (T)hab * aby) * rden;
-      cx<T>* __restrict__ carry = J.comp[field];
-      if (carry) {
-        const cx<T> lo = carry[idx];
-        const T yx = ix - lo.x, yy = iy - lo.y;
-        const T tx = v.x + yx, ty = v.y + yy;
-        cx<T> nlo = mk<T>((tx - v.x) - yx, (ty - v.y) - yy);
-        if (J.problem == MLV_PROB_DDC && field != 0 && m == 0) nlo = czero<T>();
-        carry[idx] = nlo;
-        nv = mk<T>(tx, ty);
+      const double lin = coeff * (J.fx2 * (double)(n * n) + J.fz2 * (double)(m * m));
+      const double den = 1.0 - c2 * lin;
+      const cx<T> v = vv[u];
+      cx<T> nv;
+      const bool masked = ddc && field != 0 && m == 0;
+      if constexpr (sizeof(T) == 8) {
+        // (1 + (1-a) dt L) v + AB, divided by (1 - a dt L)  (Integrator.py:58-63)
+        const T num = (T)(1.0 + c1 * lin);
+        const T rden = (T)(1.0 / den);
+        nv = mk<T>((num * v.x + (T)hab * abx) * rden, (num * v.y + (T)hab * aby) * rden);
       } else {
-        nv = mk<T>(v.x + ix, v.y + iy);
+        // The same update written as an increment, v + (dt L v + AB)/(1 - a dt L),
+        // so the rounding falls on the small change, not on (1 + (1-a) dt L).
+        // With a carry array the addition is Kahan-compensated: increments
+        // below half an ulp of v are kept instead of being dropped every step.
+        const T a = (T)(dt * lin);
+        const T rden = (T)(1.0 / den);
+        const T ix = (a * v.x + (T)hab * abx) * rden, iy = (a * v.y + (T)hab * aby) * rden;
+        if (carry) {
+          const T yx = ix - lo[u].x, yy = iy - lo[u].y;
+          const T tx = v.x + yx, ty = v.y + yy;
+          carry[idx] = masked ? czero<T>() : mk<T>((tx - v.x) - yx, (ty - v.y) - yy);
+          nv = mk<T>(tx, ty);
+        } else {
+          nv = mk<T>(v.x + ix, v.y + iy);
+        }
       }
+      if (masked) nv = czero<T>();
+      sout[idx] = nv;
     }
-    if (J.problem == MLV_PROB_DDC && field != 0 && m == 0) nv = czero<T>();
-    sout[idx] = nv;
   }
 }
 
@@ -369,9 +467,11 @@ __global__ void __launch_bounds__(kNT) k_stepC(const JobDev<T>* __restrict__ job
                                                int step, const cx<T>* __restrict__ tw) {
   extern __shared__ char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
-  const int j = find_job(pref, njobs, blockIdx.x);
+  const int lb = xcd_logical(pref[njobs]);
+  if (lb < 0) return;
+  const int j = find_job(pref, njobs, lb);
   const JobDev<T>& J = jobs[j];
-  const int local = blockIdx.x - pref[j];
+  const int local = lb - pref[j];
   const int par = (J.par0 + step) & 1;
   const int cur = ring(J.curr0, step, J.q);
   switch (J.lognx) {
@@ -404,12 +504,14 @@ __global__ void k_set_dt(JobDev<T>* jobs, int j, double dt) {
     }                                                                                     \
   } while (0)
 
+static inline unsigned xcd_grid(int n) { return (unsigned)((n + kXcds - 1) / kXcds * kXcds); }
+
 template <typename T, int V>
 static void launchB(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int wphys,
                     const cx<T>* tw, hipStream_t s) {
   if (bl.nB[stage][V] == 0) return;
   MLV_SET_LDS_ONCE((k_stepB<T, V>), bl.ldsB[V]);
-  hipLaunchKernelGGL((k_stepB<T, V>), dim3(bl.nB[stage][V]), dim3(kNT), bl.ldsB[V], s, jobs,
+  hipLaunchKernelGGL((k_stepB<T, V>), dim3(xcd_grid(bl.nB[stage][V])), dim3(kNT), bl.ldsB[V], s, jobs,
                      (const int*)bl.prefB[stage][V], bl.njobs, stage, wphys, tw);
 }
 
@@ -419,7 +521,7 @@ hipError_t fused_stage(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, 
   if (ev) (void)hipEventRecord(ev[0], s);
   if (bl.nA[stage] > 0) {
     MLV_SET_LDS_ONCE(k_stepA<T>, bl.ldsA);
-    hipLaunchKernelGGL(k_stepA<T>, dim3(bl.nA[stage]), dim3(kNT), bl.ldsA, s, jobs,
+    hipLaunchKernelGGL(k_stepA<T>, dim3(xcd_grid(bl.nA[stage])), dim3(kNT), bl.ldsA, s, jobs,
                        (const int*)bl.prefA[stage], bl.njobs, stage, step, tw);
   }
   if (ev) (void)hipEventRecord(ev[1], s);
@@ -431,7 +533,7 @@ hipError_t fused_stage(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, 
   if (ev) (void)hipEventRecord(ev[2], s);
   if (bl.nC[stage] > 0) {
     MLV_SET_LDS_ONCE(k_stepC<T>, bl.ldsC);
-    hipLaunchKernelGGL(k_stepC<T>, dim3(bl.nC[stage]), dim3(kNT), bl.ldsC, s, jobs,
+    hipLaunchKernelGGL(k_stepC<T>, dim3(xcd_grid(bl.nC[stage])), dim3(kNT), bl.ldsC, s, jobs,
                        (const int*)bl.prefC[stage], bl.njobs, stage, step, tw);
   }
   if (ev) (void)hipEventRecord(ev[3], s);

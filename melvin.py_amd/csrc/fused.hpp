@@ -114,6 +114,18 @@ template <> struct BProg<BV_MHD2> {
   }
 };
 
+// Columns per block of the x-pass kernels: every LDS slot, but at most one
+// per thread, so that each thread keeps a single column (kNT % cols == 0).
+template <int LOGN, int P>
+__host__ __device__ constexpr int fused_cols() {
+  return fft_slots<LOGN, kNT, P>() < kNT ? fft_slots<LOGN, kNT, P>() : kNT;
+}
+template <typename T>
+inline int fused_cols_host(int logn) {
+  const int s = slots_for<T>(logn);
+  return s < kNT ? s : kNT;
+}
+
 // Rows per z-pass block: as many as fit the LDS slots, at most 8.
 __host__ __device__ constexpr int brows(int nin, int neq, int slots) {
   int best = 1;

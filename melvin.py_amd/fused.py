@@ -115,7 +115,7 @@ class Job:
         if len(fields) != self.nfield:
             raise ValueError(f"expected {self.nfield} fields")
         for i, f in enumerate(fields):
-            t = torch.as_tensor(np.asarray(f) if not isinstance(f, torch.Tensor) else f)
+            t = f if isinstance(f, torch.Tensor) else torch.as_tensor(np.asarray(f))
             if tuple(t.shape) != self.spectral_shape:
                 raise ValueError(f"field {i} has shape {tuple(t.shape)}, expected "
                                  f"{self.spectral_shape}")
