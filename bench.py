@@ -94,6 +94,76 @@ def cpu_baseline(seconds=12.0):
             os.sched_setaffinity(0, old)
 
 
+def ddc_leg(m, args):
+    """Double-diffusive convection batch: ``ddc_jobs`` independent runs on
+    square grids drawn by seed from {128, 256, 512, 1024}^2, density ratio
+    R0 ~ U[1.2, 2], Pr = 7, tau = 1/Le = 0.01, domain side Ra_T^(1/4) with
+    Ra_T = 1e5 (the reference's nondimensionalisation has unit length
+    d = (kappa_T nu / g alpha T_z)^(1/4), double_diffusive_convection.py:186-207),
+    seeded T/S noise 1e-3, AB2 semi-implicit, FP32, advanced with a captured
+    HIP graph.  Also reports one 1024^2 run's FP32-vs-FP64 omega error."""
+    import torch
+
+    fused, initial = m.fused, m.initial
+    rng = np.random.default_rng(0)
+    L = 1e5 ** 0.25
+    dt = 1e-3
+    sizes = rng.choice([128, 256, 512, 1024], size=args.ddc_jobs)
+    r0s = rng.uniform(1.2, 2.0, size=args.ddc_jobs)
+
+    def job(n, r0, seed, dtype):
+        j = fused.Job("ddc", int(n), int(n), L, L, dtype=dtype, dt=dt,
+                      coeff=[7.0, 1.0, 0.01], param=[7.0, float(r0)])
+        j.set_state(list(initial.ddc_noise(int(n), L, seed=seed)))
+        return j
+
+    jobs = [job(n, r0, i, "f32") for i, (n, r0) in enumerate(zip(sizes, r0s))]
+    b = fused.Batch(jobs)
+    chunk = 20
+    b.graph_step(chunk)  # capture + warm-up
+    torch.cuda.synchronize()
+    nrep = max(1, args.ddc_steps // chunk)
+    t0 = time.perf_counter()
+    for _ in range(nrep):
+        b.graph_step(chunk)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    steps = nrep * chunk
+    pts = sum(int(n) * int(n) for n in sizes)
+    # precision check on the first 1024^2 run (or the largest)
+    k = int(np.argmax(sizes))
+    a32 = job(sizes[k], r0s[k], k, "f32")
+    a64 = job(sizes[k], r0s[k], k, "f64")
+    for a in (a32, a64):
+        bb = fused.Batch([a])
+        bb.step(steps + chunk)
+        bb.close()
+    w32, w64 = a32.state()[0], a64.state()[0]
+    err = float(np.linalg.norm(w32 - w64) / max(np.linalg.norm(w64), 1e-300))
+    b.close()
+    counts = {int(n): int((sizes == n).sum()) for n in (128, 256, 512, 1024)}
+    res = {"jobs": args.ddc_jobs, "grids": counts, "steps": steps,
+           "ms_per_step": el * 1e3 / steps, "value": pts * steps / el,
+           "unit": "grid-pt-steps/s", "graph": True,
+           "rel_err_fp32_vs_fp64_omega_largest": err}
+    print(f"[bench] ddc batch: {res}", file=sys.stderr, flush=True)
+    return res
+
+
+def serving_leg(m, args):
+    """Serving: seeded Poisson arrivals at 20 jobs/s for ``serving_seconds``
+    (vortex pair or resistive tearing, grids from {256..2048}^2, 100-5000
+    steps, FP32), admitted by HBM budget and continuously batched."""
+    S = m.serving
+    reqs = S.poisson_requests(rate=20.0, duration=args.serving_seconds, seed=0)
+    srv = S.Server(hbm_budget=96 << 30, chunk=16)
+    stats = srv.run(reqs, realtime=True)
+    stats.update(arrival_rate_per_s=20.0, arrival_window_s=args.serving_seconds,
+                 offered_grid_pt_steps=sum(r.work for r in reqs))
+    print(f"[bench] serving: {stats}", file=sys.stderr, flush=True)
+    return stats
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +173,11 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--ddc-jobs", type=int, default=64,
+                    help="double-diffusive batch leg: number of simulations (0 = skip)")
+    ap.add_argument("--ddc-steps", type=int, default=500)
+    ap.add_argument("--serving-seconds", type=float, default=8.0,
+                    help="serving leg: length of the Poisson arrival window (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (a known-size kernel "
                          "for scaling hardware byte counters)")
@@ -185,6 +260,8 @@ def main():
     dom = int(np.argmax(kms32))
     launch_bytes = kb[names[dom]] * args.batch
     achieved = launch_bytes / (kms32[dom] * 1e-3) / 1e9
+    ddc = ddc_leg(m, args) if args.ddc_jobs else None
+    serving = serving_leg(m, args) if args.serving_seconds > 0 else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -214,6 +291,8 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes_per_launch": launch_bytes},
         "cpu_baseline": cpu,
+        "ddc_batch": ddc,
+        "serving": serving,
     }
     print(json.dumps(out))
     b32.close()

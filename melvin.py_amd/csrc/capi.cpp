@@ -56,16 +56,35 @@ int get_twiddles(int device, int dtype, const void** out) {
   std::lock_guard<std::mutex> lk(g_tw_mu);
   Twiddles& t = g_tw[device];
   if (!t.f64) {
-    std::vector<double> d(2 * NMAX);
-    std::vector<float> f(2 * NMAX);
+    // main table exp(-2 pi i k / NMAX), then the per-pass tables of every
+    // length (fft.hpp: fft_pass_table), all from long-double angles
+    const int total = fft_twiddle_entries();
+    std::vector<double> d(2 * (size_t)total);
+    std::vector<float> f(2 * (size_t)total);
     const long double pi = 3.141592653589793238462643383279502884L;
-    for (int k = 0; k < NMAX; ++k) {
-      const long double a = 2.0L * pi * (long double)k / (long double)NMAX;
-      const long double c = cosl(a), s = -sinl(a);
-      d[2 * k] = (double)c;
-      d[2 * k + 1] = (double)s;
-      f[2 * k] = (float)c;
-      f[2 * k + 1] = (float)s;
+    auto put = [&](int idx, long double num, long double den) {
+      const long double a = 2.0L * pi * num / den;
+      const long double c = cosl(a), sn = -sinl(a);
+      d[2 * (size_t)idx] = (double)c;
+      d[2 * (size_t)idx + 1] = (double)sn;
+      f[2 * (size_t)idx] = (float)c;
+      f[2 * (size_t)idx + 1] = (float)sn;
+    };
+    for (int k = 0; k < NMAX; ++k) put(k, k, NMAX);
+    for (int logn = 3; logn <= LOG_NMAX; ++logn) {
+      int ns = 0;
+      while (ns < logn) {
+        const int lr = fft_pass_logr(logn, ns);
+        if (ns > 0) {
+          const int off = fft_pass_table(logn, ns);
+          const int NS = 1 << ns, R = 1 << lr;
+          for (int r = 1; r < R; ++r)
+            for (int jm = 0; jm < NS; ++jm)
+              put(off + (r - 1) * NS + jm, (long double)((long long)jm * r),
+                  (long double)(NS * R));
+        }
+        ns += lr;
+      }
     }
     HIP_TRY(hipMalloc(&t.f64, d.size() * sizeof(double)), "hipMalloc twiddles");
     HIP_TRY(hipMalloc(&t.f32, f.size() * sizeof(float)), "hipMalloc twiddles");

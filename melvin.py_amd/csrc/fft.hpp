@@ -77,7 +77,47 @@ struct Dft<1, DIR, T> {
   __device__ __forceinline__ static void run(cx<T>*) {}
 };
 
+// Per-pass twiddle tables.  After the NMAX-entry table exp(-2 pi i k/NMAX)
+// the twiddle buffer holds, for every length 2^logn (logn = 3..LOG_NMAX) and
+// every pass with Ns > 1, the (R-1) x Ns table  T[r-1][jm] = W_{Ns R}^{jm r},
+// r = 1..R-1, jm = 0..Ns-1.  A pass's butterflies on consecutive lanes have
+// consecutive jm, so each twiddle load of a wave reads consecutive entries.
+__host__ __device__ constexpr int fft_pass_logr(int logn, int logns) {
+  return logn - logns >= 4 ? 4 : logn - logns;
+}
+__host__ __device__ constexpr int fft_tables_size(int logn) {
+  int size = 0, ns = 0;
+  while (ns < logn) {
+    const int lr = fft_pass_logr(logn, ns);
+    if (ns > 0) size += ((1 << lr) - 1) << ns;
+    ns += lr;
+  }
+  return size;
+}
+// Offset (in complex entries from the start of the twiddle buffer) of the
+// table of the pass of length 2^logn whose sub-transform length is 2^logns.
+__host__ __device__ constexpr int fft_pass_table(int logn, int logns) {
+  int off = NMAX;
+  for (int l = 3; l < logn; ++l) off += fft_tables_size(l);
+  int ns = 0;
+  while (ns < logns) {
+    const int lr = fft_pass_logr(logn, ns);
+    if (ns > 0) off += ((1 << lr) - 1) << ns;
+    ns += lr;
+  }
+  return off;
+}
+__host__ __device__ constexpr int fft_twiddle_entries() {
+  int n = NMAX;
+  for (int l = 3; l <= LOG_NMAX; ++l) n += fft_tables_size(l);
+  return n;
+}
+
 // One Stockham pass: radix 2^LOGR, sub-transform length Ns = 2^LOGNS.
+// Element i of a slot is at fft_pad(i) = i + i/16.  When the stride between
+// a butterfly's R inputs (or outputs) is a multiple of 16 the padded
+// positions are a fixed stride apart, so every access is one base address
+// plus a compile-time offset.
 template <typename T, int LOGN, int LOGR, int LOGNS, int DIR, int NT, int P>
 __device__ __forceinline__ void fft_pass(cx<T>* __restrict__ lds, int active,
                                          const cx<T>* __restrict__ tw) {
@@ -87,6 +127,11 @@ __device__ __forceinline__ void fft_pass(cx<T>* __restrict__ lds, int active,
   constexpr int BF = N / R;          // butterflies per transform
   constexpr int ITEMS = P / R;       // butterflies per thread
   constexpr int LDSN = fft_ldsn(N);
+  constexpr bool RSEP = (BF % 16) == 0;
+  constexpr int RSTR = BF + BF / 16;
+  constexpr bool WSEP = (NS % 16) == 0 || (NS == 1 && R == 16);
+  constexpr int WSTR = NS == 1 ? 1 : NS + NS / 16;
+  constexpr int TOFF = LOGNS > 0 ? fft_pass_table(LOGN, LOGNS) : 0;
   const int tid = threadIdx.x;
   cx<T> v[ITEMS][R];
 #pragma unroll
@@ -96,13 +141,23 @@ __device__ __forceinline__ void fft_pass(cx<T>* __restrict__ lds, int active,
     const int j = w % BF;
     if (f < active) {
       const cx<T>* src = lds + f * LDSN;
+      if constexpr (RSEP) {
+        const cx<T>* p = src + fft_pad(j);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[it][r] = src[fft_pad(j + r * BF)];
-      if (NS > 1) {
+        for (int r = 0; r < R; ++r) v[it][r] = p[r * RSTR];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[it][r] = src[fft_pad(j + r * BF)];
+      }
+      if constexpr (NS > 1) {
         const int jm = j & (NS - 1);
+        const cx<T>* t = tw + TOFF + jm;
 #pragma unroll
-        for (int r = 1; r < R; ++r)
-          v[it][r] = cmul<T>(v[it][r], twiddle<T, DIR>(tw, jm * r, LOGNS + LOGR));
+        for (int r = 1; r < R; ++r) {
+          cx<T> wr = t[(r - 1) * NS];
+          if (DIR > 0) wr.y = -wr.y;
+          v[it][r] = cmul<T>(v[it][r], wr);
+        }
       }
       Dft<R, DIR, T>::run(v[it]);
     }
@@ -117,8 +172,14 @@ __device__ __forceinline__ void fft_pass(cx<T>* __restrict__ lds, int active,
       cx<T>* dst = lds + f * LDSN;
       const int jm = j & (NS - 1);
       const int base = (j - jm) * R + jm;
+      if constexpr (WSEP) {
+        cx<T>* p = dst + fft_pad(base);
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[fft_pad(base + r * NS)] = v[it][r];
+        for (int r = 0; r < R; ++r) p[r * WSTR] = v[it][r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst[fft_pad(base + r * NS)] = v[it][r];
+      }
     }
   }
   __syncthreads();

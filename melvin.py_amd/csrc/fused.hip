@@ -306,38 +306,92 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   const int cme = threadIdx.x % TC;
   const T szc = modk<T>(tw, c0 + cme, J.lognz, order, J.dz);
   const bool cok = cme < ncol;
+  const size_t col = (size_t)(c0 + cme);
+  cx<T>* __restrict__ slot = lds + cme * LDSN;
+  if constexpr (sizeof(T) == 4) {
+    // 1. Px column into the slot
 #pragma unroll 1
-  for (int i0 = 0; i0 < ITER; i0 += U) {
-    cx<T> pa[U], pb[U], pz[U], pa2[U], pb2[U];
+    for (int i0 = 0; i0 < ITER; i0 += U) {
+      cx<T> pv[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int x = (threadIdx.x + (i0 + u) * kNT) / TC;
-      const size_t col = (size_t)(c0 + cme);
-      if (cok) {
-        pa[u] = pxc[(size_t)((x + 1) & (N - 1)) * nm + col];
-        pb[u] = pxc[(size_t)((x - 1) & (N - 1)) * nm + col];
-        pz[u] = pzc[(size_t)x * nm + col];
-        if (order != 2) {
-          pa2[u] = pxc[(size_t)((x + 2) & (N - 1)) * nm + col];
-          pb2[u] = pxc[(size_t)((x - 2) & (N - 1)) * nm + col];
-        }
+      for (int u = 0; u < U; ++u) {
+        const int x = (threadIdx.x + (i0 + u) * kNT) / TC;
+        pv[u] = cok ? pxc[(size_t)x * nm + col] : czero<T>();
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int x = (threadIdx.x + (i0 + u) * kNT) / TC;
+        slot[fft_pad(x)] = pv[u];
       }
     }
+    __syncthreads();
+    // 2. x stencil from LDS plus i sz Pz, held in registers, then 3. stored
+    cx<T> dv[ITER];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int x = (threadIdx.x + (i0 + u) * kNT) / TC;
-      cx<T> v = czero<T>();
-      if (cok) {
+    for (int i0 = 0; i0 < ITER; i0 += U) {
+      cx<T> zv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int x = (threadIdx.x + (i0 + u) * kNT) / TC;
+        zv[u] = cok ? pzc[(size_t)x * nm + col] : czero<T>();
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int x = (threadIdx.x + (i0 + u) * kNT) / TC;
+        auto at = [&](int d) { return slot[fft_pad((x + d) & (N - 1))]; };
         cx<T> d;
         if (order == 2) {
-          d = mk<T>((pa[u].x - pb[u].x) * ihx, (pa[u].y - pb[u].y) * ihx);
+          const cx<T> a = at(1), b = at(-1);
+          d = mk<T>((a.x - b.x) * ihx, (a.y - b.y) * ihx);
         } else {
-          d = mk<T>((T(-0.25) * pa2[u].x + T(2) * pa[u].x - T(2) * pb[u].x + T(0.25) * pb2[u].x) * ihx,
-                    (T(-0.25) * pa2[u].y + T(2) * pa[u].y - T(2) * pb[u].y + T(0.25) * pb2[u].y) * ihx);
+          const cx<T> a2 = at(2), a1 = at(1), b1 = at(-1), b2 = at(-2);
+          d = mk<T>((T(-0.25) * a2.x + T(2) * a1.x - T(2) * b1.x + T(0.25) * b2.x) * ihx,
+                    (T(-0.25) * a2.y + T(2) * a1.y - T(2) * b1.y + T(0.25) * b2.y) * ihx);
         }
-        v = mk<T>(d.x - szc * pz[u].y, d.y + szc * pz[u].x);
+        dv[i0 + u] = cok ? mk<T>(d.x - szc * zv[u].y, d.y + szc * zv[u].x) : czero<T>();
       }
-      lds[cme * LDSN + fft_pad(x)] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < ITER; ++i) {
+      const int x = (threadIdx.x + i * kNT) / TC;
+      slot[fft_pad(x)] = dv[i];
+    }
+  } else {
+    // double precision: the register file is the limit, so the stencil
+    // reads its neighbours straight from global memory (L2 hits)
+#pragma unroll 1
+    for (int i0 = 0; i0 < ITER; i0 += U) {
+      cx<T> pa[U], pb[U], pz[U], pa2[U], pb2[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int x = (threadIdx.x + (i0 + u) * kNT) / TC;
+        if (cok) {
+          pa[u] = pxc[(size_t)((x + 1) & (N - 1)) * nm + col];
+          pb[u] = pxc[(size_t)((x - 1) & (N - 1)) * nm + col];
+          pz[u] = pzc[(size_t)x * nm + col];
+          if (order != 2) {
+            pa2[u] = pxc[(size_t)((x + 2) & (N - 1)) * nm + col];
+            pb2[u] = pxc[(size_t)((x - 2) & (N - 1)) * nm + col];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int x = (threadIdx.x + (i0 + u) * kNT) / TC;
+        cx<T> v = czero<T>();
+        if (cok) {
+          cx<T> d;
+          if (order == 2) {
+            d = mk<T>((pa[u].x - pb[u].x) * ihx, (pa[u].y - pb[u].y) * ihx);
+          } else {
+            d = mk<T>((T(-0.25) * pa2[u].x + T(2) * pa[u].x - T(2) * pb[u].x + T(0.25) * pb2[u].x) * ihx,
+                      (T(-0.25) * pa2[u].y + T(2) * pa[u].y - T(2) * pb[u].y + T(0.25) * pb2[u].y) * ihx);
+          }
+          v = mk<T>(d.x - szc * pz[u].y, d.y + szc * pz[u].x);
+        }
+        slot[fft_pad(x)] = v;
+      }
     }
   }
   __syncthreads();
@@ -362,6 +416,8 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   const bool ddc = J.problem == MLV_PROB_DDC;
   cx<T>* __restrict__ carry = sizeof(T) == 4 ? J.comp[field] : nullptr;
   const int prev1 = (cur + q - 1) % q, prev2 = (cur + q - 2) % q, prev3 = (cur + q - 3) % q;
+  const float fcoeff = (float)coeff, ffx2 = (float)J.fx2, ffz2 = (float)J.fz2;
+  const float fdt = (float)dt, fc2 = (float)c2;
 #pragma unroll 1
   for (int rb = r0; rb < rows; rb += U2 * RSTEP) {
     cx<T> vv[U2], h1[U2], h2[U2], h3[U2], lo[U2], e1[U2], e2[U2];
@@ -428,23 +484,24 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
         abx = T(55) * rhs.x - T(59) * h1[u].x + T(37) * h2[u].x - T(9) * h3[u].x;
         aby = T(55) * rhs.y - T(59) * h1[u].y + T(37) * h2[u].y - T(9) * h3[u].y;
       }
-      const double lin = coeff * (J.fx2 * (double)(n * n) + J.fz2 * (double)(m * m));
-      const double den = 1.0 - c2 * lin;
       const cx<T> v = vv[u];
       cx<T> nv;
       const bool masked = ddc && field != 0 && m == 0;
       if constexpr (sizeof(T) == 8) {
         // (1 + (1-a) dt L) v + AB, divided by (1 - a dt L)  (Integrator.py:58-63)
+        const double lin = coeff * (J.fx2 * (double)(n * n) + J.fz2 * (double)(m * m));
         const T num = (T)(1.0 + c1 * lin);
-        const T rden = (T)(1.0 / den);
+        const T rden = (T)(1.0 / (1.0 - c2 * lin));
         nv = mk<T>((num * v.x + (T)hab * abx) * rden, (num * v.y + (T)hab * aby) * rden);
       } else {
         // The same update written as an increment, v + (dt L v + AB)/(1 - a dt L),
-        // so the rounding falls on the small change, not on (1 + (1-a) dt L).
-        // With a carry array the addition is Kahan-compensated: increments
-        // below half an ulp of v are kept instead of being dropped every step.
-        const T a = (T)(dt * lin);
-        const T rden = (T)(1.0 / den);
+        // so the rounding falls on the small change, not on (1 + (1-a) dt L);
+        // the factors therefore only need single precision.  With a carry
+        // array the addition is Kahan-compensated: increments below half an
+        // ulp of v are kept instead of being dropped every step.
+        const float lin = fcoeff * (ffx2 * (float)(n * n) + ffz2 * (float)(m * m));
+        const T a = fdt * lin;
+        const T rden = __builtin_amdgcn_rcpf(1.0f - fc2 * lin);
         const T ix = (a * v.x + (T)hab * abx) * rden, iy = (a * v.y + (T)hab * aby) * rden;
         if (carry) {
           const T yx = ix - lo[u].x, yy = iy - lo[u].y;
