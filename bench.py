@@ -62,6 +62,27 @@ def kernel_bytes(nx, nz, nf, q, cbytes, carry=False):
     return {"x_inverse": a, "z_pass": b, "x_forward_update": c}
 
 
+KERNEL_SYMBOLS = {"x_inverse": "mlv::k_stepA<float>", "z_pass": "mlv::k_stepB<float, 1>",
+                  "x_forward_update": "mlv::k_stepC<float>"}
+
+
+def measured_traffic(kernel, batch):
+    """HBM bytes per launch of ``kernel`` read by rocprofv3's FETCH_SIZE and
+    WRITE_SIZE counters (calibrated against a plain copy of known size) for
+    this workload at this batch size, from the committed counter summary;
+    None when no summary covers the configuration."""
+    path = os.path.join(ROOT, "profiles", f"kh{NX}_batch{batch}_hbm_traffic.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    block = text.split("```json", 1)[-1].split("```", 1)[0]
+    try:
+        return json.loads(block)["bytes_per_launch"].get(KERNEL_SYMBOLS[kernel])
+    except (ValueError, KeyError):
+        return None
+
+
 def cpu_baseline(seconds=12.0):
     """The oracle's FP64 KH step (NumPy/pocketfft, one thread) on the host."""
     from oracle import melvin_oracle as O
@@ -289,7 +310,8 @@ def main():
         "kernel_ms_per_launch": dict(zip(names, kms32)),
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "algorithmic_bytes_per_launch": launch_bytes},
+                     "traffic": measured_traffic(names[dom], args.batch),
+                     "algorithmic_bytes_per_launch": launch_bytes},
         "cpu_baseline": cpu,
         "ddc_batch": ddc,
         "serving": serving,

@@ -545,11 +545,84 @@ def step_rbc(p, st):
     return ux, uz
 
 
+def fdz_implicit_bands(p, coeff, dt, alpha):
+    """Bands of I - alpha*dt*coeff*L_n for every x mode n, where L_n is the
+    FD-z Laplacian -kx^2 + D2z on interior points; the wall rows are
+    identity.  Extension of this build: the reference has no semi-implicit
+    FD-z scheme (Integrator.py:78-83).  Parity unpinned against the
+    reference; pinned by a dense solve and the explicit limit in tests."""
+    nn, nz, dz = p.nn, p.nz, p.dz
+    n = np.arange(nn)[:, None]
+    k2 = (n * 2 * np.pi / p.lx) ** 2
+    a = alpha * dt * coeff
+    diag = (1.0 + a * (k2 + 2.0 / dz ** 2)) * np.ones((nn, nz))
+    lower = np.full((nn, nz), -a / dz ** 2)
+    upper = np.full((nn, nz), -a / dz ** 2)
+    diag[:, 0] = diag[:, -1] = 1.0
+    upper[:, 0] = lower[:, -1] = 0.0
+    return lower, diag, upper
+
+
+def fdz_laplacian_apply(p, f):
+    """L_n f = -kx^2 f + D2z f on interior points, zero on the walls (the
+    operator of snabla2 in FD-z, SpatialDifferentiator.py:36-40, order 2)."""
+    n = np.arange(p.nn)[:, None]
+    out = -((n * 2 * np.pi / p.lx) ** 2) * f
+    out[:, 0] = 0.0
+    out[:, -1] = 0.0
+    out[:, 1:-1] += (f[:, 2:] - 2 * f[:, 1:-1] + f[:, :-2]) / p.dz ** 2
+    return out
+
+
+def integrate_fdz_semi_implicit(p, var, hist: History, coeff, dt, alpha):
+    """theta-scheme diffusion for FD-z (order-2 z differences): solve
+    (I - a dt c L) v_new = (I + (1-a) dt c L) v + AB(hist) mode by mode,
+    identity wall rows, then advance the ring."""
+    rhs = var + (1 - alpha) * dt * coeff * fdz_laplacian_apply(p, var) + adams_bashforth(hist, dt)
+    lo, di, up = fdz_implicit_bands(p, coeff, dt, alpha)
+    out = thomas(lo, di, up, rhs)
+    hist.advance()
+    return out
+
+
+def _rbc_walls(p, st):
+    w, t, ps = st["w"], st["tmp"], st["psi"]
+    k = 1 if p.spatial_derivative_order == 2 else 2
+    w[1:, :k] = 0.0
+    w[1:, -k:] = 0.0
+    ps[1:, :k] = 0.0
+    ps[1:, -k:] = 0.0
+    t[0, :k] = 1.0
+    t[0, -k:] = 0.0
+    t[1:, :k] = 0.0
+    t[1:, -k:] = 0.0
+    ps[0, :] = 0.0
+    w[0, :] = 0.0
+
+
+def step_rbc_semi_implicit(p, st):
+    """Rayleigh-Benard with theta-implicit diffusion (this build's extension
+    for stiff FD-z diffusion): the advection and buoyancy terms of
+    rayleigh_benard_convection.py:101-110 in the AB ring, diffusion by
+    integrate_fdz_semi_implicit, then the same wall writes (113-142)."""
+    Pr, Ra = p.Pr, p.Ra
+    psi, ux, uz, uz_spec, psi_phys = velocity_from_vorticity(p, st["w"])
+    st["psi"] = psi
+    T0 = st["tmp"].copy()
+    st["dw"].set(-vec_dot_nabla(p, st["w"], ux, uz) - Pr * Ra * sddx(p, T0))
+    st["w"] = integrate_fdz_semi_implicit(p, st["w"], st["dw"], Pr, st["dt"], p.alpha)
+    st["dtmp"].set(-vec_dot_nabla(p, T0, ux, uz))
+    st["tmp"] = integrate_fdz_semi_implicit(p, T0, st["dtmp"], 1.0, st["dt"], p.alpha)
+    _rbc_walls(p, st)
+    return ux, uz
+
+
 PROBLEMS = {
     "vorticity": (("w",), step_vorticity),
     "ddc": (("w", "tmp", "xi"), step_ddc),
     "mhd": (("w", "j"), step_mhd),
     "rbc": (("w", "tmp"), step_rbc),
+    "rbc_si": (("w", "tmp"), step_rbc_semi_implicit),
 }
 
 

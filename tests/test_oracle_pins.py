@@ -257,3 +257,27 @@ def test_taylor_green_decay_semi_implicit():
     g = (1 + 0.49 * 1e-2 * L) / (1 - 0.51 * 1e-2 * L)
     np.testing.assert_allclose(st["w"], w0 * g ** 20, atol=1e-13)
     assert g ** 20 == pytest.approx(np.exp(-2 * nu * 0.2), rel=1e-6)
+
+
+def test_fdz_semi_implicit_solves_its_system():
+    """The theta-implicit FD-z step (an extension; the reference has none)
+    solves (I - a dt c L) v_new = (I + (1-a) dt c L) v + AB exactly, mode by
+    mode, and for a = 0 it is the forward-Euler/AB update of the same L."""
+    p = O.make_params(dict(nx=16, nz=9, lx=2.0, lz=1.0, discretisation=["spectral", "fdm"]))
+    rng = np.random.default_rng(7)
+    v = rng.standard_normal(p.spectral_shape) + 1j * rng.standard_normal(p.spectral_shape)
+    hv = rng.standard_normal((2,) + p.spectral_shape) + 0j
+    for alpha in (0.51, 0.0):
+        h = O.History(2, p.spectral_shape)
+        h.data[:] = hv
+        h.curr = 1
+        ab = O.adams_bashforth(h, 1e-3)
+        out = O.integrate_fdz_semi_implicit(p, v.copy(), h, 0.7, 1e-3, alpha)
+        for n in range(p.nn):
+            L = np.zeros((p.nz, p.nz))
+            for j in range(1, p.nz - 1):
+                L[j, j - 1] = L[j, j + 1] = 1 / p.dz ** 2
+                L[j, j] = -2 / p.dz ** 2 - (2 * np.pi * n / p.lx) ** 2
+            A = np.eye(p.nz) - alpha * 1e-3 * 0.7 * L
+            b = v[n] + (1 - alpha) * 1e-3 * 0.7 * (L @ v[n]) + ab[n]
+            np.testing.assert_allclose(out[n], np.linalg.solve(A, b), rtol=1e-12, atol=1e-14)
