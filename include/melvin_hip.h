@@ -194,6 +194,71 @@ int mlv_batch_profile(mlv_batch* batch, int enable);
  * kernels since profiling was enabled (synchronises with the events). */
 int mlv_batch_profile_read(mlv_batch* batch, double* ms, long long* launches);
 
+/* ---------------------------------------------------------------------
+ * Fused Fourier-x / finite-difference-z steps over a table of jobs: the
+ * Rayleigh-Benard loop body (rayleigh_benard_convection.py:95-145) --
+ * LaplacianSolver.solve in FD-z (LaplacianSolver.py:70-78),
+ * calc_velocity_from_vorticity (utility.py:62-79), vec_dot_nabla of omega
+ * and T (Variable.py:119-128), buoyancy, Integrator.integrate and the
+ * spectral wall writes -- for omega and T on (nn, nz) spectral rows.
+ * ------------------------------------------------------------------- */
+/* Integrator._explicit, the reference's FD-z scheme (Integrator.py:53-56,
+ * 78-83) */
+#define MLV_FDZ_EXPLICIT 0
+/* theta-implicit second-order z diffusion solved per x mode (FP64
+ * tridiagonal), Adams-Bashforth advection and buoyancy; no reference
+ * counterpart (the reference has no semi-implicit FD-z path) */
+#define MLV_FDZ_SEMI_IMPLICIT 1
+/* Adams-Bashforth predictor and Adams-Moulton corrector (the formulas of
+ * Integrator.py:5-33; the reference never calls the corrector) around the
+ * same theta-implicit diffusion */
+#define MLV_FDZ_PC 2
+
+typedef struct mlv_fdz_batch mlv_fdz_batch;
+
+typedef struct mlv_fdz_job_desc {
+  int nx, nz;          /* nx a power of two in [8, 4096]; nz in [5, 768] */
+  double lx, lz;       /* dz = lz / nz (Parameters.py:82) */
+  int fd_order;        /* 2 or 4: z differences of advection and explicit
+                          diffusion, the x difference, ux = -pddz(psi) */
+  int ab_order;        /* 2 or 4 */
+  int scheme;          /* MLV_FDZ_* */
+  double alpha, dt, Pr, Ra;
+  void* state[2][2];   /* [buffer][omega, T] spectral (nn, nz) */
+  void* hist[2];       /* ring per field: ab_order x (nn, nz) */
+  void* comp[2];       /* F32 only, optional: per-field residual of the state
+                          (value = state - comp), zero-initialised, or NULL */
+  void* psi[2];        /* optional, per buffer: the walled psi of that
+                          buffer's omega, written by the step that produced
+                          it (psi of a step = psi[buffer the step read]) */
+  void* work;          /* mlv_fdz_work_bytes() bytes */
+  void* phys[4];       /* optional physical ux, uz, omega, T (nx, nz) of the
+                          state a step starts from, or NULL */
+  int curr;            /* ring slot the next step writes */
+  int parity;          /* state buffer holding the current state */
+} mlv_fdz_job_desc;
+
+int mlv_fdz_work_bytes(int nx, int nz, int dtype, size_t* bytes);
+int mlv_fdz_batch_create(mlv_fdz_batch** batch, int dtype, const mlv_fdz_job_desc* jobs,
+                         int njobs);
+int mlv_fdz_batch_destroy(mlv_fdz_batch* batch);
+/* Velocity (and psi) of every job's current state; run by the first step
+ * and needed again only after a state is replaced from outside. */
+int mlv_fdz_batch_prime(mlv_fdz_batch* batch, void* stream);
+/* Run nsteps steps; write_phys = 1 also writes the physical fields of the
+ * last of them.  Each step reads dt from the job table. */
+int mlv_fdz_batch_step(mlv_fdz_batch* batch, int nsteps, int write_phys, void* stream);
+/* As mlv_batch_graph_step, for FD-z batches. */
+int mlv_fdz_batch_graph_step(mlv_fdz_batch* batch, int nsteps, void* stream);
+/* Set one job's dt (and its implicit operators) for the following steps
+ * (Integrator.override_dt, Integrator.py:46-48). */
+int mlv_fdz_batch_set_dt(mlv_fdz_batch* batch, int job, double dt, void* stream);
+long long mlv_fdz_batch_steps(const mlv_fdz_batch* batch);
+/* Kernel timing (measurement only): milliseconds and launch counts of the
+ * column (x transform) and row (z solve / update) kernels. */
+int mlv_fdz_batch_profile(mlv_fdz_batch* batch, int enable);
+int mlv_fdz_batch_profile_read(mlv_fdz_batch* batch, double* ms, long long* launches);
+
 #ifdef __cplusplus
 }
 #endif

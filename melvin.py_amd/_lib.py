@@ -27,6 +27,7 @@ FD_DDX, FD_DDZ, FD_D2DZ2 = range(3)
 SCHEME_SEMI_IMPLICIT, SCHEME_EXPLICIT = 0, 1
 RED_MAX, RED_SUM, RED_SUMSQ2, RED_DOT = range(4)
 PROB_VORTICITY, PROB_VORT_SCALAR, PROB_DDC, PROB_MHD = range(4)
+FDZ_EXPLICIT, FDZ_SEMI_IMPLICIT, FDZ_PC = range(3)
 
 # Every symbol include/melvin_hip.h declares (checked by the tests).
 EXPORTS = (
@@ -36,6 +37,10 @@ EXPORTS = (
     "mlv_reduce", "mlv_combine", "mlv_velocity", "mlv_job_work_bytes", "mlv_batch_create",
     "mlv_batch_destroy", "mlv_batch_step", "mlv_batch_graph_step", "mlv_batch_set_dt",
     "mlv_batch_steps", "mlv_batch_profile", "mlv_batch_profile_read",
+    "mlv_fdz_work_bytes", "mlv_fdz_batch_create", "mlv_fdz_batch_destroy",
+    "mlv_fdz_batch_prime", "mlv_fdz_batch_step", "mlv_fdz_batch_graph_step",
+    "mlv_fdz_batch_set_dt", "mlv_fdz_batch_steps", "mlv_fdz_batch_profile",
+    "mlv_fdz_batch_profile_read",
 )
 
 
@@ -65,6 +70,32 @@ class JobDesc(ctypes.Structure):
         ("work", ctypes.c_void_p),
         ("psi", ctypes.c_void_p),
         ("phys", ctypes.c_void_p * 8),
+        ("curr", ctypes.c_int),
+        ("parity", ctypes.c_int),
+    ]
+
+
+class FdzJobDesc(ctypes.Structure):
+    """Mirror of ``mlv_fdz_job_desc``."""
+
+    _fields_ = [
+        ("nx", ctypes.c_int),
+        ("nz", ctypes.c_int),
+        ("lx", ctypes.c_double),
+        ("lz", ctypes.c_double),
+        ("fd_order", ctypes.c_int),
+        ("ab_order", ctypes.c_int),
+        ("scheme", ctypes.c_int),
+        ("alpha", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("Pr", ctypes.c_double),
+        ("Ra", ctypes.c_double),
+        ("state", (ctypes.c_void_p * 2) * 2),
+        ("hist", ctypes.c_void_p * 2),
+        ("comp", ctypes.c_void_p * 2),
+        ("psi", ctypes.c_void_p * 2),
+        ("work", ctypes.c_void_p),
+        ("phys", ctypes.c_void_p * 4),
         ("curr", ctypes.c_int),
         ("parity", ctypes.c_int),
     ]
@@ -102,6 +133,17 @@ def _declare(lib):
         "mlv_batch_steps": ([vp], ctypes.c_longlong),
         "mlv_batch_profile": ([vp, i], i),
         "mlv_batch_profile_read": ([vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_longlong)], i),
+        "mlv_fdz_work_bytes": ([i, i, i, ctypes.POINTER(sz)], i),
+        "mlv_fdz_batch_create": ([pp, i, ctypes.POINTER(FdzJobDesc), i], i),
+        "mlv_fdz_batch_destroy": ([vp], i),
+        "mlv_fdz_batch_prime": ([vp, vp], i),
+        "mlv_fdz_batch_step": ([vp, i, i, vp], i),
+        "mlv_fdz_batch_graph_step": ([vp, i, vp], i),
+        "mlv_fdz_batch_set_dt": ([vp, i, d, vp], i),
+        "mlv_fdz_batch_steps": ([vp], ctypes.c_longlong),
+        "mlv_fdz_batch_profile": ([vp, i], i),
+        "mlv_fdz_batch_profile_read": ([vp, ctypes.POINTER(d),
+                                        ctypes.POINTER(ctypes.c_longlong)], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

@@ -11,13 +11,15 @@
 #include <vector>
 
 #include "../../include/melvin_hip.h"
+#include "capi_common.hpp"
 #include "fused.hpp"
 #include "ops.hpp"
 #include "transforms.hpp"
 
 using namespace mlv;
 
-namespace {
+namespace mlv {
+namespace abi {
 
 thread_local std::string g_err = "";
 
@@ -31,11 +33,6 @@ int hip_fail(hipError_t e, const char* what) {
   return MLV_ERR_HIP;
 }
 
-#define HIP_TRY(call, what)                    \
-  do {                                         \
-    hipError_t e_ = (call);                    \
-    if (e_ != hipSuccess) return hip_fail(e_, what); \
-  } while (0)
 
 int ilog2_exact(int n) {
   if (n <= 0 || (n & (n - 1))) return -1;
@@ -119,7 +116,10 @@ int global_partials(int device, double** out) {
   return MLV_OK;
 }
 
-}  // namespace
+}  // namespace abi
+}  // namespace mlv
+
+using namespace mlv::abi;
 
 struct mlv_plan {
   int device;

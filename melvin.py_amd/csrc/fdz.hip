@@ -1,0 +1,632 @@
+// Fused Fourier-x / FD-z step kernels over a job table (see fdz.hpp).
+#include <math.h>
+
+#include "fdz.hpp"
+#include "grid.hpp"
+
+namespace mlv {
+
+template <typename T>
+__device__ __forceinline__ double2 d2(cx<T> v) {
+  return make_double2((double)v.x, (double)v.y);
+}
+template <typename T>
+__device__ __forceinline__ cx<T> t2(double2 v) {
+  return mk<T>((T)v.x, (T)v.y);
+}
+
+// --------------------------------------------------------------------------
+// Tridiagonal solve of one row by one wavefront, FP64.
+//
+// The system has identity first and last rows and constant interior bands
+// (lower = upper = lo).  fac holds, for element j, (cp_j, 1/den_j) of the
+// Thomas factorisation, stored chunk-major: entry i * 64 + lane is element
+// j = lane * K + i.  Lane l owns elements [l K, l K + K).
+//   forward   d_j = (r_j - lower_j d_{j-1}) / den_j
+//   backward  x_j = d_j - cp_j x_{j+1},  x_{nz-1} = d_{nz-1}
+// Each recurrence is an affine map per element; a lane composes the maps of
+// its chunk, an inclusive shuffle scan composes them across lanes, and the
+// lane then replays its chunk from the value entering it.  buf holds the
+// right-hand side on entry and the solution on exit.
+// --------------------------------------------------------------------------
+template <int KT>
+__device__ __forceinline__ void wave_tridiag(double2* __restrict__ buf,
+                                             const double2* __restrict__ fac, double lo, int nz,
+                                             int K, int lane) {
+  const int j0 = lane * K;
+  double2 d[KT], fc[KT];
+#pragma unroll
+  for (int i = 0; i < KT; ++i) {
+    const int j = j0 + i;
+    const bool ok = i < K && j < nz;
+    fc[i] = ok ? fac[i * 64 + lane] : make_double2(0.0, 1.0);
+    d[i] = ok ? buf[j] : make_double2(0.0, 0.0);
+  }
+  // forward elimination
+  double A = 1.0, Bx = 0.0, By = 0.0;
+#pragma unroll
+  for (int i = 0; i < KT; ++i) {
+    const int j = j0 + i;
+    if (i < K && j < nz) {
+      const double l = (j == 0 || j == nz - 1) ? 0.0 : lo;
+      const double m = -l * fc[i].y;
+      A = m * A;
+      Bx = m * Bx + d[i].x * fc[i].y;
+      By = m * By + d[i].y * fc[i].y;
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double Ao = __shfl_up(A, off, 64);
+    const double Xo = __shfl_up(Bx, off, 64);
+    const double Yo = __shfl_up(By, off, 64);
+    if (lane >= off) {
+      Bx = A * Xo + Bx;
+      By = A * Yo + By;
+      A = A * Ao;
+    }
+  }
+  double px = __shfl_up(Bx, 1, 64), py = __shfl_up(By, 1, 64);
+  if (lane == 0) px = py = 0.0;
+#pragma unroll
+  for (int i = 0; i < KT; ++i) {
+    const int j = j0 + i;
+    if (i < K && j < nz) {
+      const double l = (j == 0 || j == nz - 1) ? 0.0 : lo;
+      d[i].x = (d[i].x - l * px) * fc[i].y;
+      d[i].y = (d[i].y - l * py) * fc[i].y;
+      px = d[i].x;
+      py = d[i].y;
+    }
+  }
+  // back substitution
+  A = 1.0;
+  Bx = 0.0;
+  By = 0.0;
+#pragma unroll
+  for (int i = KT - 1; i >= 0; --i) {
+    const int j = j0 + i;
+    if (i < K && j < nz) {
+      const double c = fc[i].x;
+      A = -c * A;
+      Bx = d[i].x - c * Bx;
+      By = d[i].y - c * By;
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double Ao = __shfl_down(A, off, 64);
+    const double Xo = __shfl_down(Bx, off, 64);
+    const double Yo = __shfl_down(By, off, 64);
+    if (lane + off < 64) {
+      Bx = A * Xo + Bx;
+      By = A * Yo + By;
+      A = A * Ao;
+    }
+  }
+  double xx = __shfl_down(Bx, 1, 64), xy = __shfl_down(By, 1, 64);
+  if (lane == 63) xx = xy = 0.0;
+#pragma unroll
+  for (int i = KT - 1; i >= 0; --i) {
+    const int j = j0 + i;
+    if (i < K && j < nz) {
+      xx = d[i].x - fc[i].x * xx;
+      xy = d[i].y - fc[i].x * xy;
+      d[i] = make_double2(xx, xy);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < KT; ++i) {
+    const int j = j0 + i;
+    if (i < K && j < nz) buf[j] = d[i];
+  }
+}
+
+// First z difference of the row in buf at j (pddz, SpatialDifferentiator.py:
+// 91-104, 159-185): zero on the end points the stencil does not reach.
+__device__ __forceinline__ double2 ddz_row(const double2* __restrict__ b, int j, int nz, int order,
+                                           double ih) {
+  if (order == 2) {
+    if (j < 1 || j > nz - 2) return make_double2(0.0, 0.0);
+    return make_double2((b[j + 1].x - b[j - 1].x) * ih, (b[j + 1].y - b[j - 1].y) * ih);
+  }
+  if (j < 2 || j > nz - 3) return make_double2(0.0, 0.0);
+  return make_double2(
+      (-0.25 * b[j + 2].x + 2.0 * b[j + 1].x - 2.0 * b[j - 1].x + 0.25 * b[j - 2].x) * ih,
+      (-0.25 * b[j + 2].y + 2.0 * b[j + 1].y - 2.0 * b[j - 1].y + 0.25 * b[j - 2].y) * ih);
+}
+
+// Second z difference (pd2dz2, SpatialDifferentiator.py:106-128).
+__device__ __forceinline__ double2 d2dz2_row(const double2* __restrict__ b, int j, int nz,
+                                             int order, double idz2) {
+  if (order == 2) {
+    if (j < 1 || j > nz - 2) return make_double2(0.0, 0.0);
+    return make_double2((b[j + 1].x - 2.0 * b[j].x + b[j - 1].x) * idz2,
+                        (b[j + 1].y - 2.0 * b[j].y + b[j - 1].y) * idz2);
+  }
+  if (j < 2 || j > nz - 3) return make_double2(0.0, 0.0);
+  const double c1 = -1.0 / 12, c2 = 4.0 / 3, c3 = -5.0 / 2;
+  return make_double2(
+      (c1 * b[j + 2].x + c2 * b[j + 1].x + c3 * b[j].x + c2 * b[j - 1].x + c1 * b[j - 2].x) * idz2,
+      (c1 * b[j + 2].y + c2 * b[j + 1].y + c3 * b[j].y + c2 * b[j - 1].y + c1 * b[j - 2].y) * idz2);
+}
+
+// --------------------------------------------------------------------------
+// Row kernel body: one wavefront, mode n, field f (0 omega, 1 T).  Element-
+// wise work uses the coalesced layout j = lane + 64 i.  The wave owns two
+// LDS rows of FP64 complex values: buf (operands with neighbours, and the
+// tridiagonal solves) and vrow (the state v_n the update starts from).
+// --------------------------------------------------------------------------
+template <typename T, int KT>
+__device__ void fdz_row(const FdzJob<T>& J, int stage, int n, int f, int step,
+                        double2* __restrict__ buf, double2* __restrict__ vrow) {
+  const int lane = threadIdx.x & 63;
+  const int Z = J.nz, K = J.K;
+  const int par = (J.par0 + step) & 1;
+  const size_t row = (size_t)n * Z;
+  const int order = J.fd_order;
+  const int kw = order == 2 ? 1 : 2;  // width of the wall band
+  const double dz = J.dz;
+  const double ih = order == 2 ? 1.0 / (2.0 * dz) : 1.0 / (3.0 * dz);
+  if (stage != FDZ_PROLOGUE) {
+    const int q = J.q;
+    const int cur = ring(J.curr0, step, q);
+    const cx<T>* __restrict__ qx = J.qw[2 * f] + row;
+    const cx<T>* __restrict__ qz = J.qw[2 * f + 1] + row;
+    const cx<T>* __restrict__ vin = J.state[par][f] + row;
+    const cx<T>* __restrict__ cin = J.comp[f] ? J.comp[f] + row : nullptr;
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      const int j = lane + 64 * i;
+      if (i < K && j < Z) {
+        buf[j] = d2<T>(qz[j]);
+        double2 v = d2<T>(vin[j]);
+        if (cin) {
+          const double2 c = d2<T>(cin[j]);
+          v.x -= c.x;
+          v.y -= c.y;
+        }
+        vrow[j] = v;
+      }
+    }
+    wave_sync();
+    double sx;
+    {
+      const double th = 2.0 * M_PI * (double)n / (double)J.nx;
+      sx = order == 2 ? sin(th) / J.dx : (8.0 * sin(th) - sin(2.0 * th)) / (6.0 * J.dx);
+    }
+    // -Pr Ra i (2 pi n / lx) T  (rayleigh_benard_convection.py:101-104)
+    const double buoy = f == 0 ? J.Pr * J.Ra * (J.fx * (double)n) : 0.0;
+    const cx<T>* __restrict__ tsrc = (stage == FDZ_CORR ? J.star[1] : J.state[par][1]) + row;
+    const double coeff = f == 0 ? J.Pr : 1.0;
+    const double k2 = J.fx2 * (double)n * (double)n;  // -(2 pi n / lx)^2
+    const double idz2 = 1.0 / (dz * dz);
+    const bool expl = J.scheme == MLV_FDZ_EXPLICIT;
+    const int ws = stage == FDZ_CORR ? (cur + 1) % q : cur;  // ring slot written
+    const int s1 = (ws + q - 1) % q, s2 = (ws + q - 2) % q, s3 = (ws + q - 3) % q;
+    const size_t ssz = (size_t)J.nn * Z;
+    cx<T>* __restrict__ hist = J.hist[f];
+    const double dt = J.dt;
+    double2 inc[KT];
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      const int j = lane + 64 * i;
+      inc[i] = make_double2(0.0, 0.0);
+      if (i < K && j < Z) {
+        const size_t e = row + j;
+        // issue the global loads first
+        const double2 a = d2<T>(qx[j]);
+        const double2 t = f == 0 ? d2<T>(tsrc[j]) : make_double2(0.0, 0.0);
+        const double2 h1 = d2<T>(hist[(size_t)s1 * ssz + e]);
+        double2 h2 = make_double2(0.0, 0.0), h3 = make_double2(0.0, 0.0);
+        if (q == 4) {
+          h2 = d2<T>(hist[(size_t)s2 * ssz + e]);
+          h3 = d2<T>(hist[(size_t)s3 * ssz + e]);
+        }
+        // advection N = -(i sx Qx + Dz Qz), buoyancy
+        const double2 dd = ddz_row(buf, j, Z, order, ih);
+        double2 h0 = make_double2(sx * a.y - dd.x, -(sx * a.x) - dd.y);
+        h0.x += buoy * t.y;
+        h0.y -= buoy * t.x;
+        const double2 v = vrow[j];
+        double2 lv = make_double2(0.0, 0.0);
+        if (expl) {
+          // diffusion added into the stored RHS (Integrator.py:53-56):
+          // coeff * (sd2dx2 + pd2dz2) v
+          const double2 dzz = d2dz2_row(vrow, j, Z, order, idz2);
+          h0.x += coeff * (k2 * v.x + dzz.x);
+          h0.y += coeff * (k2 * v.y + dzz.y);
+        } else if (j > 0 && j < Z - 1) {
+          // L v = -kx^2 v + second-order D2z v on interior points, 0 on walls
+          lv = make_double2(k2 * v.x + (vrow[j + 1].x - 2.0 * v.x + vrow[j - 1].x) * idz2,
+                            k2 * v.y + (vrow[j + 1].y - 2.0 * v.y + vrow[j - 1].y) * idz2);
+        }
+        hist[(size_t)ws * ssz + e] = t2<T>(h0);
+        double2 E;
+        if (stage == FDZ_CORR) {  // Adams-Moulton (Integrator.py:20-33)
+          if (q == 2) {
+            E = make_double2(dt / 2 * (h0.x + h1.x), dt / 2 * (h0.y + h1.y));
+          } else {
+            E = make_double2(dt / 24 * (9 * h0.x + 19 * h1.x - 5 * h2.x + h3.x),
+                             dt / 24 * (9 * h0.y + 19 * h1.y - 5 * h2.y + h3.y));
+          }
+        } else {  // Adams-Bashforth (Integrator.py:5-18)
+          if (q == 2) {
+            E = make_double2(dt / 2 * (3 * h0.x - h1.x), dt / 2 * (3 * h0.y - h1.y));
+          } else {
+            E = make_double2(dt / 24 * (55 * h0.x - 59 * h1.x + 37 * h2.x - 9 * h3.x),
+                             dt / 24 * (55 * h0.y - 59 * h1.y + 37 * h2.y - 9 * h3.y));
+          }
+        }
+        inc[i] = expl ? E : make_double2(dt * coeff * lv.x + E.x, dt * coeff * lv.y + E.y);
+      }
+    }
+    wave_sync();
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      const int j = lane + 64 * i;
+      if (i < K && j < Z) buf[j] = inc[i];
+    }
+    if (!expl) {
+      // increment form of the theta scheme:
+      //   (I - a dt c L) delta = dt c L v + E,  v_new = v + delta
+      wave_sync();
+      wave_tridiag<KT>(buf, J.fac[1 + f] + (size_t)n * K * 64, J.lo[1 + f], Z, K, lane);
+    }
+    wave_sync();
+    // new state and wall writes (rayleigh_benard_convection.py:113-142);
+    // buf[j] becomes -omega_new for the Poisson solve (same lane, same j)
+    cx<T>* __restrict__ out = (stage == FDZ_PRED ? J.star[f] : J.state[par ^ 1][f]) + row;
+    cx<T>* __restrict__ cout = (stage != FDZ_PRED && J.comp[f]) ? J.comp[f] + row : nullptr;
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      const int j = lane + 64 * i;
+      if (i < K && j < Z) {
+        const double2 dl = buf[j], v0 = vrow[j];
+        double2 v = make_double2(v0.x + dl.x, v0.y + dl.y);
+        bool wall;
+        if (f == 0) {
+          wall = n == 0 || j < kw || j >= Z - kw;
+          if (wall) v = make_double2(0.0, 0.0);
+        } else {
+          wall = j < kw || j >= Z - kw;
+          if (wall) v = make_double2((j < kw && n == 0) ? 1.0 : 0.0, 0.0);
+        }
+        const cx<T> hi = t2<T>(v);
+        out[j] = hi;
+        if (cout) {
+          // value = hi - residual, with the residual rounded once
+          cout[j] = wall ? czero<T>() : mk<T>((T)((double)hi.x - v.x), (T)((double)hi.y - v.y));
+        }
+        // the Poisson solve sees the value the next evaluation reads
+        const double2 wv = (cout || sizeof(T) == 8) ? v : d2<T>(hi);
+        buf[j] = make_double2(-wv.x, -wv.y);
+      }
+    }
+  } else {
+    const cx<T>* __restrict__ vin = J.state[par][0] + row;
+    const cx<T>* __restrict__ cin = J.comp[0] ? J.comp[0] + row : nullptr;
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      const int j = lane + 64 * i;
+      if (i < K && j < Z) {
+        double2 w = d2<T>(vin[j]);
+        if (cin) {
+          const double2 c = d2<T>(cin[j]);
+          w.x -= c.x;
+          w.y -= c.y;
+        }
+        buf[j] = make_double2(-w.x, -w.y);
+      }
+    }
+  }
+  if (f != 0) return;
+  // psi = solve(-omega) (utility.py:65, LaplacianSolver.py:70-78),
+  // ux_s = -pddz(psi), uz_s = i kx psi (utility.py:66-79)
+  wave_sync();
+  wave_tridiag<KT>(buf, J.fac[0] + (size_t)n * K * 64, J.lo[0], Z, K, lane);
+  wave_sync();
+  const int vs = stage == FDZ_PRED ? 2 : (stage == FDZ_PROLOGUE ? par : par ^ 1);
+  cx<T>* __restrict__ ux = J.vel[vs][0] + row;
+  cx<T>* __restrict__ uz = J.vel[vs][1] + row;
+  cx<T>* __restrict__ ps = vs < 2 && J.psi[vs] ? J.psi[vs] + row : nullptr;
+  const double kx = J.fx * (double)n;
+#pragma unroll
+  for (int i = 0; i < KT; ++i) {
+    const int j = lane + 64 * i;
+    if (i < K && j < Z) {
+      const double2 p = buf[j];
+      const double2 dd = ddz_row(buf, j, Z, order, ih);
+      ux[j] = t2<T>(make_double2(-dd.x, -dd.y));
+      uz[j] = t2<T>(make_double2(-(kx * p.y), kx * p.x));
+      if (ps) ps[j] = (n == 0 || j < kw || j >= Z - kw) ? czero<T>() : t2<T>(p);
+    }
+  }
+}
+
+template <typename T, int KT>
+__global__ void __launch_bounds__(64 * kFdzWaves)
+    k_fdz_rows(const FdzJob<T>* __restrict__ jobs, const int* __restrict__ pref, int njobs,
+               int phase, int step, int zstride) {
+  extern __shared__ char smem_raw[];
+  const int lb = xcd_logical(pref[njobs]);
+  if (lb < 0) return;
+  const int jb = find_job(pref, njobs, lb);
+  const FdzJob<T>& J = jobs[jb];
+  const int wave = threadIdx.x >> 6;
+  const int lw = (lb - pref[jb]) * kFdzWaves + wave;
+  int stage, nwf;
+  if (phase < 0) {
+    stage = FDZ_PROLOGUE;
+    nwf = 1;
+  } else {
+    stage = J.scheme == MLV_FDZ_PC ? (phase == 0 ? FDZ_PRED : FDZ_CORR) : FDZ_MAIN;
+    nwf = 2;
+  }
+  if (lw >= nwf * J.nn || J.kt != KT) return;
+  const int n = lw / nwf, f = lw % nwf;
+  double2* buf = reinterpret_cast<double2*>(smem_raw) + (size_t)(2 * wave) * zstride;
+  fdz_row<T, KT>(J, stage, n, f, step, buf, buf + zstride);
+}
+
+// --------------------------------------------------------------------------
+// Column kernel: TC z columns per block, two LDS slots per column.
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void herm_pack(cx<T>* __restrict__ z, int k, int N, cx<T> a, cx<T> b) {
+  // Spectra of two real sequences (k < nn kept) packed into one complex
+  // sequence a + i b; irfft ignores the imaginary part of the k = 0 term.
+  if (k == 0) {
+    z[0] = mk<T>(a.x, b.x);
+  } else {
+    z[fft_pad(k)] = mk<T>(a.x - b.y, a.y + b.x);
+    z[fft_pad(N - k)] = mk<T>(a.x + b.y, b.x - a.y);
+  }
+}
+
+template <typename T, int LOGN>
+__device__ __forceinline__ void fdz_cols_body(const FdzJob<T>& J, int local, int phase, int wphys,
+                                              int step, const cx<T>* __restrict__ tw,
+                                              cx<T>* lds) {
+  constexpr int N = 1 << LOGN;
+  constexpr int P = fft_points_col<T>(LOGN);
+  constexpr int SL = fft_slots<LOGN, kNT, P>();
+  if constexpr (SL < 2) {
+    return;  // rejected on the host
+  } else {
+    constexpr int TC = (SL / 2) > kNT ? kNT : SL / 2;
+    constexpr int LDSN = fft_ldsn(N);
+    const int Z = J.nz, nn = J.nn;
+    const int par = (J.par0 + step) & 1;
+    const bool corr = J.scheme == MLV_FDZ_PC && phase == 1;
+    const int j0 = local * TC;
+    const int ncol = min(TC, Z - j0);
+    const int vs = corr ? 2 : par;
+    const cx<T>* __restrict__ ux = J.vel[vs][0];
+    const cx<T>* __restrict__ uz = J.vel[vs][1];
+    const cx<T>* __restrict__ w = corr ? J.star[0] : J.state[par][0];
+    const cx<T>* __restrict__ t = corr ? J.star[1] : J.state[par][1];
+    const int nzero = N - 2 * nn + 1;
+    for (int e = threadIdx.x; e < 2 * ncol * nzero; e += kNT) {
+      const int s = e / nzero, k = nn + e % nzero;
+      lds[s * LDSN + fft_pad(k)] = czero<T>();
+    }
+    constexpr int U = 4;
+    const int nfill = TC * nn;
+#pragma unroll 1
+    for (int e0 = threadIdx.x; e0 < nfill; e0 += U * kNT) {
+      cx<T> a[U], b[U], c[U], d[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * kNT;
+        const int cc = e % TC, k = e / TC;
+        if (e < nfill && cc < ncol) {
+          const size_t idx = (size_t)k * Z + j0 + cc;
+          a[u] = ux[idx];
+          b[u] = uz[idx];
+          c[u] = w[idx];
+          d[u] = t[idx];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = e0 + u * kNT;
+        const int cc = e % TC, k = e / TC;
+        if (e < nfill && cc < ncol) {
+          herm_pack<T>(lds + (2 * cc) * LDSN, k, N, a[u], b[u]);
+          herm_pack<T>(lds + (2 * cc + 1) * LDSN, k, N, c[u], d[u]);
+        }
+      }
+    }
+    __syncthreads();
+    fft_lds<T, LOGN, +1, kNT, P>(lds, 2 * ncol, tw);
+    const bool wp = wphys && !corr;
+    for (int e = threadIdx.x; e < ncol * N; e += kNT) {
+      const int cc = e / N, x = e % N;
+      cx<T>* za = lds + (2 * cc) * LDSN + fft_pad(x);
+      cx<T>* zb = lds + (2 * cc + 1) * LDSN + fft_pad(x);
+      const cx<T> v = *za, s = *zb;  // (ux, uz), (omega, T)
+      if (wp) {
+        const size_t o = (size_t)x * Z + j0 + cc;
+        if (J.phys[0]) J.phys[0][o] = v.x;
+        if (J.phys[1]) J.phys[1][o] = v.y;
+        if (J.phys[2]) J.phys[2][o] = s.x;
+        if (J.phys[3]) J.phys[3][o] = s.y;
+      }
+      *za = mk<T>(v.x * s.x, v.y * s.x);
+      *zb = mk<T>(v.x * s.y, v.y * s.y);
+    }
+    __syncthreads();
+    fft_lds<T, LOGN, -1, kNT, P>(lds, 2 * ncol, tw);
+    const T hs = (T)(0.5 / (double)N);
+    for (int e = threadIdx.x; e < nfill; e += kNT) {
+      const int cc = e % TC, k = e / TC;
+      if (cc >= ncol) continue;
+      const size_t idx = (size_t)k * Z + j0 + cc;
+#pragma unroll
+      for (int fi = 0; fi < 2; ++fi) {
+        const cx<T>* z = lds + (2 * cc + fi) * LDSN;
+        const cx<T> y = z[fft_pad(k)];
+        const cx<T> ym = z[fft_pad((N - k) & (N - 1))];
+        J.qw[2 * fi][idx] = mk<T>(hs * (y.x + ym.x), hs * (y.y - ym.y));
+        J.qw[2 * fi + 1][idx] = mk<T>(hs * (y.y + ym.y), hs * (ym.x - y.x));
+      }
+    }
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kNT)
+    k_fdz_cols(const FdzJob<T>* __restrict__ jobs, const int* __restrict__ pref, int njobs,
+               int phase, int wphys, int step, const cx<T>* __restrict__ tw) {
+  extern __shared__ char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int lb = xcd_logical(pref[njobs]);
+  if (lb < 0) return;
+  const int jb = find_job(pref, njobs, lb);
+  const FdzJob<T>& J = jobs[jb];
+  const int local = lb - pref[jb];
+  switch (J.lognx) {
+#define X(L)                                                          \
+  case L:                                                             \
+    fdz_cols_body<T, L>(J, local, phase, wphys, step, tw, lds);       \
+    break;
+    X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+#undef X
+    default:
+      break;
+  }
+}
+
+// Thomas factors of one job's operator, one thread per mode (FP64).
+//   which = 0: Poisson, interior bands [1/dz^2, -(kx^2 + 2/dz^2), 1/dz^2]
+//              (LaplacianSolver.py:21-52)
+//   which > 0: theta diffusion, [-a/dz^2, 1 + a (kx^2 + 2/dz^2), -a/dz^2]
+// First and last rows are identity rows.
+__global__ void k_fdz_factors(double2* __restrict__ fac, int nn, int nz, int K, double fxabs,
+                              double dz, int which, double a) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  double2* F = fac + (size_t)n * K * 64;
+  const double k = (double)n * fxabs;
+  const double k2 = k * k;
+  const double idz2 = 1.0 / (dz * dz);
+  double lo, dg;
+  if (which == 0) {
+    lo = idz2;
+    dg = -(k2 + 2.0 / (dz * dz));
+  } else {
+    lo = -a * idz2;
+    dg = 1.0 + a * (k2 + 2.0 / (dz * dz));
+  }
+  for (int e = 0; e < K * 64; ++e) F[e] = make_double2(0.0, 1.0);
+  F[0] = make_double2(0.0, 1.0);
+  double cprev = 0.0;
+  for (int j = 1; j < nz; ++j) {
+    const bool last = j == nz - 1;
+    const double l = last ? 0.0 : lo;
+    const double dj = last ? 1.0 : dg;
+    const double up = last ? 0.0 : lo;
+    const double rd = 1.0 / (dj - l * cprev);
+    const double c = up * rd;
+    F[(j % K) * 64 + j / K] = make_double2(c, rd);
+    cprev = c;
+  }
+}
+
+template <typename T>
+__global__ void k_fdz_set_dt(FdzJob<T>* jobs, int j, double dt, double lo_w, double lo_t) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    jobs[j].dt = dt;
+    jobs[j].lo[1] = lo_w;
+    jobs[j].lo[2] = lo_t;
+  }
+}
+
+// --------------------------------------------------------------------------
+// Host side
+// --------------------------------------------------------------------------
+template <typename T, int V>
+static void launch_rows(const FdzJob<T>* jobs, const FdzLaunch& fl, const int* pref, int nblocks,
+                        int phase, int step, hipStream_t s) {
+  if (nblocks <= 0) return;
+  constexpr int KT = kFdzKts[V];
+  MLV_SET_LDS_ONCE((k_fdz_rows<T, KT>), fl.ldsrow);
+  hipLaunchKernelGGL((k_fdz_rows<T, KT>), dim3(xcd_grid(nblocks)), dim3(64 * kFdzWaves), fl.ldsrow,
+                     s, jobs, pref, fl.njobs, phase, step, fl.zstride);
+}
+
+template <typename T>
+static void launch_rows_all(const FdzJob<T>* jobs, const FdzLaunch& fl, int phase, int step,
+                            hipStream_t s) {
+  const int p = phase < 0 ? 2 : phase;
+  launch_rows<T, 0>(jobs, fl, fl.prefrow[p][0], fl.nrow[p][0], phase, step, s);
+  launch_rows<T, 1>(jobs, fl, fl.prefrow[p][1], fl.nrow[p][1], phase, step, s);
+  launch_rows<T, 2>(jobs, fl, fl.prefrow[p][2], fl.nrow[p][2], phase, step, s);
+  launch_rows<T, 3>(jobs, fl, fl.prefrow[p][3], fl.nrow[p][3], phase, step, s);
+  launch_rows<T, 4>(jobs, fl, fl.prefrow[p][4], fl.nrow[p][4], phase, step, s);
+  launch_rows<T, 5>(jobs, fl, fl.prefrow[p][5], fl.nrow[p][5], phase, step, s);
+  static_assert(kFdzVariants == 6, "row kernel variants");
+}
+
+template <typename T>
+hipError_t fdz_phase(const FdzJob<T>* jobs, const FdzLaunch& fl, int phase, int wphys, int step,
+                     const cx<T>* tw, hipStream_t s, hipEvent_t* ev) {
+  if (ev) (void)hipEventRecord(ev[0], s);
+  if (fl.ncol[phase] > 0) {
+    MLV_SET_LDS_ONCE(k_fdz_cols<T>, fl.ldscol);
+    hipLaunchKernelGGL(k_fdz_cols<T>, dim3(xcd_grid(fl.ncol[phase])), dim3(kNT), fl.ldscol, s,
+                       jobs, (const int*)fl.prefcol[phase], fl.njobs, phase, wphys, step, tw);
+  }
+  if (ev) (void)hipEventRecord(ev[1], s);
+  launch_rows_all<T>(jobs, fl, phase, step, s);
+  if (ev) (void)hipEventRecord(ev[2], s);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t fdz_prologue(const FdzJob<T>* jobs, const FdzLaunch& fl, int step, hipStream_t s) {
+  launch_rows_all<T>(jobs, fl, -1, step, s);
+  return hipGetLastError();
+}
+
+hipError_t fdz_factors(double2* fac, int nn, int nz, int K, double fxabs, double dz, int which,
+                       double a, hipStream_t s) {
+  hipLaunchKernelGGL(k_fdz_factors, dim3((nn + 63) / 64), dim3(64), 0, s, fac, nn, nz, K, fxabs,
+                     dz, which, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t fdz_set_dt(FdzJob<T>* jobs, int job, double dt, double lo_w, double lo_t,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_fdz_set_dt<T>, dim3(1), dim3(64), 0, s, jobs, job, dt, lo_w, lo_t);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t fdz_prepare(const FdzLaunch& fl) {
+  if (fl.ldscol) MLV_SET_LDS_ONCE(k_fdz_cols<T>, fl.ldscol);
+  if (fl.ldsrow) {
+    MLV_SET_LDS_ONCE((k_fdz_rows<T, kFdzKts[0]>), fl.ldsrow);
+    MLV_SET_LDS_ONCE((k_fdz_rows<T, kFdzKts[1]>), fl.ldsrow);
+    MLV_SET_LDS_ONCE((k_fdz_rows<T, kFdzKts[2]>), fl.ldsrow);
+    MLV_SET_LDS_ONCE((k_fdz_rows<T, kFdzKts[3]>), fl.ldsrow);
+    MLV_SET_LDS_ONCE((k_fdz_rows<T, kFdzKts[4]>), fl.ldsrow);
+    MLV_SET_LDS_ONCE((k_fdz_rows<T, kFdzKts[5]>), fl.ldsrow);
+  }
+  return hipGetLastError();
+}
+
+template hipError_t fdz_phase<float>(const FdzJob<float>*, const FdzLaunch&, int, int, int,
+                                     const cx<float>*, hipStream_t, hipEvent_t*);
+template hipError_t fdz_phase<double>(const FdzJob<double>*, const FdzLaunch&, int, int, int,
+                                      const cx<double>*, hipStream_t, hipEvent_t*);
+template hipError_t fdz_prologue<float>(const FdzJob<float>*, const FdzLaunch&, int, hipStream_t);
+template hipError_t fdz_prologue<double>(const FdzJob<double>*, const FdzLaunch&, int, hipStream_t);
+template hipError_t fdz_set_dt<float>(FdzJob<float>*, int, double, double, double, hipStream_t);
+template hipError_t fdz_set_dt<double>(FdzJob<double>*, int, double, double, double, hipStream_t);
+template hipError_t fdz_prepare<float>(const FdzLaunch&);
+template hipError_t fdz_prepare<double>(const FdzLaunch&);
+
+}  // namespace mlv

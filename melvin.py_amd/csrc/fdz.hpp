@@ -1,0 +1,94 @@
+// Fused Fourier-x / finite-difference-z step (the Rayleigh-Benard body of
+// rayleigh_benard_convection.py:95-145) over a table of jobs.
+//
+// Spectral state (nn, nz) complex per field (omega, T): row = x mode
+// kx in [0, nn), column = z grid point.  One evaluation of the explicit
+// terms ("phase") is two kernels:
+//
+//   cols  per block of z columns: Hermitian-packed inverse x transforms of
+//         (ux_s + i uz_s) and (omega + i T), the four products ux*f, uz*f
+//         in physical space, two packed forward x transforms, keep
+//         kx < nn, scale 1/nx  ->  Qx_f, Qz_f.
+//   rows  one wavefront per (x mode, field): the flux divergence
+//         D = i sx(kx) Qx + Dz(Qz) (the periodic x difference is its exact
+//         modified wavenumber; the z difference acts along the row, and both
+//         commute with the x transform), buoyancy -Pr Ra i kx T for omega,
+//         the Adams-Bashforth / Adams-Moulton ring, the update (the
+//         reference's explicit one, Integrator.py:53-56, or theta-implicit
+//         diffusion solved per mode), the spectral wall writes
+//         (rayleigh_benard_convection.py:113-142), and for omega the FP64
+//         Poisson solve psi = solve(-omega) of the new state
+//         (LaplacianSolver.py:70-78) with ux_s = -Dz(psi), uz_s = i kx psi
+//         (utility.py:62-79): the velocity of the next evaluation.
+//
+// Schemes: EXPLICIT and SEMI_IMPLICIT run one phase per step; PC runs two
+// (AB predictor into a scratch state, AM corrector from the old state).
+// The tridiagonal solves run in FP64 whatever the state precision: each
+// lane of the wave owns a contiguous chunk of the row, Thomas's two
+// first-order recurrences are composed into affine maps per chunk and
+// scanned across the 64 lanes with shuffles.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "../../include/melvin_hip.h"
+#include "transforms.hpp"
+
+namespace mlv {
+
+enum FdzStage : int { FDZ_MAIN = 0, FDZ_PRED = 1, FDZ_CORR = 2, FDZ_PROLOGUE = 3 };
+constexpr int kFdzWaves = 4;   // wavefronts (mode, field) per row block
+// Row elements per lane (K = ceil(nz / 64)) the row kernel is compiled for;
+// a job runs in the smallest variant >= its K.  nz <= 64 * 12: beyond that
+// the per-lane arrays no longer fit the register file.
+constexpr int kFdzVariants = 6;
+constexpr int kFdzKts[kFdzVariants] = {1, 2, 4, 6, 9, 12};
+constexpr int kFdzMaxK = 12;
+
+template <typename T>
+struct FdzJob {
+  int nx, nz, lognx, nn, K, kt, fd_order, scheme, q, curr0, par0;
+  double fx, fx2, dx, dz, dt, alpha, Pr, Ra;
+  double lo[3];            // interior off-diagonal of each tridiagonal operator
+  cx<T>* state[2][2];      // [buffer][omega, T]
+  cx<T>* star[2];          // predicted state (PC)
+  cx<T>* hist[2];          // ring per field: q x (nn, nz)
+  cx<T>* comp[2];          // single precision: residual of the state (value = state - comp)
+  cx<T>* psi[2];           // per buffer: walled psi of that buffer's omega, or NULL
+  cx<T>* vel[3][2];        // [buffer 0, 1, predicted][ux_s, uz_s]
+  cx<T>* qw[4];            // Qx_omega, Qz_omega, Qx_T, Qz_T
+  T* phys[4];              // optional physical ux, uz, omega, T
+  double2* fac[3];         // (cp, 1/den) per operator: Poisson, theta(omega), theta(T)
+};
+
+struct FdzLaunch {
+  int njobs;
+  int ncol[2];                       // column blocks of phases 0 and 1
+  int nrow[3][kFdzVariants];         // row blocks of phases 0, 1 and the prologue
+  size_t ldsrow, ldscol;
+  int zstride;                       // LDS elements per row of the row kernel
+  int *prefcol[2];
+  int *prefrow[3][kFdzVariants];
+};
+
+// Blocks of the column kernel per job and its columns per block.
+template <typename T>
+inline int fdz_cols_per_block(int lognx) {
+  const int s = slots_for<T>(lognx) / 2;
+  return s < 1 ? 0 : (s > kNT ? kNT : s);
+}
+
+template <typename T>
+hipError_t fdz_phase(const FdzJob<T>* jobs, const FdzLaunch& fl, int phase, int wphys, int step,
+                     const cx<T>* tw, hipStream_t s, hipEvent_t* ev = nullptr);
+template <typename T>
+hipError_t fdz_prologue(const FdzJob<T>* jobs, const FdzLaunch& fl, int step, hipStream_t s);
+// (Re)build the factor tables of one job: which = 0 Poisson, 1 theta(omega),
+// 2 theta(T); theta tables depend on dt.
+hipError_t fdz_factors(double2* fac, int nn, int nz, int K, double fxabs, double dz, int which,
+                       double a, hipStream_t s);
+template <typename T>
+hipError_t fdz_set_dt(FdzJob<T>* jobs, int job, double dt, double lo_w, double lo_t,
+                      hipStream_t s);
+template <typename T>
+hipError_t fdz_prepare(const FdzLaunch& fl);
+
+}  // namespace mlv

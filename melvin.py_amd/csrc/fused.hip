@@ -1,34 +1,8 @@
 // Fused fully-spectral step kernels over a job table (see fused.hpp).
 #include "fused.hpp"
+#include "grid.hpp"
 
 namespace mlv {
-
-__device__ __forceinline__ int find_job(const int* __restrict__ pref, int njobs, int b) {
-  int lo = 0, hi = njobs;  // pref[lo] <= b < pref[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (pref[mid] <= b)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ int ring(int c0, int step, int q) { return (c0 + step) % q; }
-
-// Workgroups of a launch are dealt to the 8 XCDs round robin (block b runs
-// on XCD b % 8).  Give each XCD a contiguous range of logical blocks, so
-// neighbouring column tiles -- which share 128-byte lines whenever a tile is
-// narrower than a line -- are fetched through the same L2.  The grid is
-// padded to a multiple of 8; returns -1 for padding blocks.
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_logical(int nlogical) {
-  const int per = (nlogical + kXcds - 1) / kXcds;
-  const int b = blockIdx.x;
-  const int l = (b % kXcds) * per + b / kXcds;
-  return l < nlogical ? l : -1;
-}
 
 // --------------------------------------------------------------------------
 // Kernel A: spectral prologue + inverse x pass
@@ -551,18 +525,6 @@ __global__ void k_set_dt(JobDev<T>* jobs, int j, double dt) {
 // --------------------------------------------------------------------------
 // Host side
 // --------------------------------------------------------------------------
-#define MLV_SET_LDS_ONCE(kern, bytes)                                                     \
-  do {                                                                                    \
-    static size_t done_ = 0;                                                              \
-    if (done_ < (bytes)) {                                                                \
-      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),                      \
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)(bytes)); \
-      done_ = (bytes);                                                                    \
-    }                                                                                     \
-  } while (0)
-
-static inline unsigned xcd_grid(int n) { return (unsigned)((n + kXcds - 1) / kXcds * kXcds); }
-
 template <typename T, int V>
 static void launchB(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int wphys,
                     const cx<T>* tw, hipStream_t s) {

@@ -78,7 +78,7 @@ class Job:
         self.hist = [torch.zeros((self.ab_order,) + shape, dtype=self.cplx, device=dev)
                      for _ in range(nf)]
         # single precision: Kahan carry of the state update (see the x-forward
-        # kernel), so the stored state is buf + carry
+        # kernel); the value held is buf - carry
         self.carry = ([torch.zeros(shape, dtype=self.cplx, device=dev) for _ in range(nf)]
                       if (compensated and not self.double) else None)
         nbytes = ctypes.c_size_t()
@@ -123,7 +123,7 @@ class Job:
             hi = t.to(self.cplx)
             self.buf[self.parity][i].copy_(hi)
             if self.carry is not None:
-                self.carry[i].copy_((t - hi.to(torch.complex128)).to(self.cplx))
+                self.carry[i].copy_((hi.to(torch.complex128) - t).to(self.cplx))
 
     def current(self):
         """Device tensors of the current spectral state."""
@@ -131,12 +131,12 @@ class Job:
 
     def state(self):
         """Current spectral state as complex128 host arrays (including the
-        single-precision carry when there is one)."""
+        single-precision carry when there is one: value = buf - carry)."""
         out = []
         for i, b in enumerate(self.current()):
             v = b.detach().cpu().numpy().astype(np.complex128)
             if self.carry is not None:
-                v = v + self.carry[i].detach().cpu().numpy().astype(np.complex128)
+                v = v - self.carry[i].detach().cpu().numpy().astype(np.complex128)
             out.append(v)
         return out
 
@@ -253,11 +253,13 @@ def cfl_dt(job, dt, cutoff=0.5):
     return dt
 
 
-def run(batch, nsteps, cfl_cadence=10, cutoff=0.5, on_cfl=None):
+def run(batch, nsteps, cfl_cadence=10, cutoff=0.5, on_cfl=None, cfl=None):
     """Advance every job ``nsteps`` loops with the reference's CFL ticker
     (Simulation.py:99-109, Ticker.py:14-25): dt is checked after loops 1,
     1+cadence, ... from that loop's velocities and applies from the next
-    loop.  Jobs need ``want_phys``; cfl_cadence=0 disables the check."""
+    loop.  Jobs need ``want_phys``; cfl_cadence=0 disables the check.
+    ``cfl(job, dt, cutoff)`` is the dt rule (default: cfl_dt)."""
+    cfl_dt_fn = cfl if cfl is not None else cfl_dt
     done = 0
     next_check = 1
     while done < nsteps:
@@ -265,7 +267,7 @@ def run(batch, nsteps, cfl_cadence=10, cutoff=0.5, on_cfl=None):
             batch.step(1, write_phys=True)
             done += 1
             for i, job in enumerate(batch.jobs):
-                new = cfl_dt(job, job.dt, cutoff)
+                new = cfl_dt_fn(job, job.dt, cutoff)
                 if new != job.dt:
                     batch.set_dt(i, new)
                 if on_cfl is not None:

@@ -617,12 +617,66 @@ def step_rbc_semi_implicit(p, st):
     return ux, uz
 
 
+def _rbc_rhs(p, w, t):
+    """Explicit RHS of the Rayleigh-Benard equations at (w, t): advection and
+    buoyancy (rayleigh_benard_convection.py:101-110).  Returns the RHS pair,
+    the streamfunction and the physical velocities."""
+    psi, ux, uz, _, _ = velocity_from_vorticity(p, w)
+    rw = -vec_dot_nabla(p, w, ux, uz) - p.Pr * p.Ra * sddx(p, t)
+    rt = -vec_dot_nabla(p, t, ux, uz)
+    return rw, rt, psi, ux, uz
+
+
+def _fdz_theta_solve(p, var, incr, coeff, dt, alpha):
+    """theta-scheme FD-z diffusion with a given explicit increment ``incr``:
+    solve (I - a dt c L) v_new = (I + (1-a) dt c L) var + incr."""
+    rhs = var + (1 - alpha) * dt * coeff * fdz_laplacian_apply(p, var) + incr
+    lo, di, up = fdz_implicit_bands(p, coeff, dt, alpha)
+    return thomas(lo, di, up, rhs)
+
+
+def step_rbc_pc(p, st):
+    """Rayleigh-Benard with an Adams-Bashforth predictor and Adams-Moulton
+    corrector (PECE) for the explicit terms and theta-implicit FD-z diffusion
+    in both stages.  Extension of this build: the reference defines the
+    AB/AM formulas (Integrator.py:5-33) but never calls the corrector.
+
+    P: N_n = RHS(v_n) into the ring slot curr; v* = theta-solve with the AB
+       increment over slots curr, curr-1, ...; wall writes on v*.
+    E: N* = RHS(v*) into slot curr+1 (the oldest slot, no longer needed).
+    C: v_{n+1} = theta-solve from v_n with the AM increment over slots
+       curr+1, curr, ... ; wall writes; the ring ends advanced once.
+    """
+    Pr = p.Pr
+    dt, a = st["dt"], p.alpha
+    w0, t0 = st["w"].copy(), st["tmp"].copy()
+    rw, rt, psi, ux, uz = _rbc_rhs(p, w0, t0)
+    st["psi"] = psi
+    hw, ht = st["dw"], st["dtmp"]
+    hw.set(rw)
+    ht.set(rt)
+    ws = _fdz_theta_solve(p, w0, adams_bashforth(hw, dt), Pr, dt, a)
+    ts = _fdz_theta_solve(p, t0, adams_bashforth(ht, dt), 1.0, dt, a)
+    star = {"w": ws, "tmp": ts, "psi": np.zeros_like(ws)}
+    _rbc_walls(p, star)
+    rw2, rt2, _, _, _ = _rbc_rhs(p, star["w"], star["tmp"])
+    hw.advance()
+    ht.advance()
+    hw.set(rw2)
+    ht.set(rt2)
+    st["w"] = _fdz_theta_solve(p, w0, adams_moulton(hw, dt), Pr, dt, a)
+    st["tmp"] = _fdz_theta_solve(p, t0, adams_moulton(ht, dt), 1.0, dt, a)
+    _rbc_walls(p, st)
+    return ux, uz
+
+
 PROBLEMS = {
     "vorticity": (("w",), step_vorticity),
     "ddc": (("w", "tmp", "xi"), step_ddc),
     "mhd": (("w", "j"), step_mhd),
     "rbc": (("w", "tmp"), step_rbc),
     "rbc_si": (("w", "tmp"), step_rbc_semi_implicit),
+    "rbc_pc": (("w", "tmp"), step_rbc_pc),
 }
 
 

@@ -74,3 +74,54 @@ def test_argument_errors_are_reported():
     assert lib.mlv_job_work_bytes(L.PROB_VORTICITY, 1, 32, 32, 0, ctypes.byref(n)) == 0
     # TGV channels: w, ux, uz + Px, Pz -> 5 * nx * nm complex64
     assert n.value == 5 * 32 * 10 * 8
+
+
+def test_fdz_job_desc_layout_matches_c(tmp_path):
+    L = _lib()
+    src = tmp_path / "layout_fdz.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "melvin_hip.h"\n'
+        "int main(void){printf(\"%zu %zu %zu %zu %zu %zu\\n\", sizeof(mlv_fdz_job_desc),"
+        " offsetof(mlv_fdz_job_desc, alpha), offsetof(mlv_fdz_job_desc, state),"
+        " offsetof(mlv_fdz_job_desc, work), offsetof(mlv_fdz_job_desc, phys),"
+        " offsetof(mlv_fdz_job_desc, parity));return 0;}\n")
+    exe = tmp_path / "layout_fdz"
+    subprocess.run(["gcc", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    vals = [int(v) for v in subprocess.check_output([str(exe)]).split()]
+    J = L.FdzJobDesc
+    assert vals == [ctypes.sizeof(J), J.alpha.offset, J.state.offset, J.work.offset,
+                    J.phys.offset, J.parity.offset]
+
+
+def test_fdz_argument_errors_are_reported():
+    """Descriptors are validated on the host before any device call."""
+    L = _lib()
+    lib = L.lib()
+    n = ctypes.c_size_t()
+    assert lib.mlv_fdz_work_bytes(32, 17, 0, ctypes.byref(n)) == 0
+    # Qx/Qz of two fields, velocity of three states, predicted state
+    assert n.value == 12 * 10 * 17 * 8
+    assert lib.mlv_fdz_work_bytes(32, 3, 0, ctypes.byref(n)) == L.MLV_ERR_ARG
+    d = L.FdzJobDesc()
+    d.nx, d.nz, d.lx, d.lz = 32, 17, 2.0, 1.0
+    d.fd_order, d.ab_order, d.scheme = 2, 2, L.FDZ_PC
+    d.alpha, d.dt, d.Pr, d.Ra = 0.51, 1e-4, 1.0, 1e5
+    dummy = ctypes.c_void_p(16)
+    for b in range(2):
+        for f in range(2):
+            d.state[b][f] = dummy
+    d.hist[0] = d.hist[1] = dummy
+    d.work = dummy
+    h = ctypes.c_void_p()
+    for field, bad, word in (("nx", 48, b"power of two"), ("nz", 769, b"nz"),
+                             ("scheme", 7, b"scheme"), ("fd_order", 3, b"fd_order"),
+                             ("ab_order", 3, b"ab_order"), ("dt", 0.0, b"positive")):
+        good = getattr(d, field)
+        setattr(d, field, bad)
+        assert lib.mlv_fdz_batch_create(ctypes.byref(h), 0, ctypes.byref(d), 1) == L.MLV_ERR_ARG
+        assert word in lib.mlv_last_error(), field
+        setattr(d, field, good)
+    d.work = None
+    assert lib.mlv_fdz_batch_create(ctypes.byref(h), 0, ctypes.byref(d), 1) == L.MLV_ERR_ARG
+    assert lib.mlv_fdz_batch_step(None, 1, 0, None) == L.MLV_ERR_ARG
+    assert lib.mlv_fdz_batch_steps(None) == -1
