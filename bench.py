@@ -171,6 +171,90 @@ def ddc_leg(m, args):
     return res
 
 
+def rbc_leg(m, args):
+    """Rayleigh-Benard, Fourier-x / FD-z: nx = 1024 (nn = 341 modes), nz = 513,
+    [0,2] x [0,1], Ra = 1e7, Pr = 1, T = 1 - z plus seeded noise 1e-3,
+    omega = 0, AB2/AM2 predictor-corrector with theta-implicit z diffusion,
+    dt starting at 1e-5 under the reference's CFL ticker (checks every 10
+    loops, Integrator.py:35-44).  FP32 transforms and state, FP64
+    tridiagonal solves; the same run in FP64 is the comparison, and an FP64
+    run from the FP32-rounded initial state measures how much the flow
+    itself amplifies a rounding-sized perturbation over the same steps.
+    Kinetic energy (utility.py:42-59) and the Nusselt number 1 + <uz T>
+    (heat flux in units of the conductive flux) are FP64 device reductions."""
+    import torch
+
+    fdz, hip, initial = m.fdz, m.hip, m.initial
+    nx, nz, steps = 1024, 513, args.rbc_steps
+    w0, t0 = initial.rbc_conduction(nx, nz, 2.0, 1.0, seed=0, amp=1e-3)
+
+    def run(dtype, w, t, timed=False):
+        job = fdz.FdzJob(nx, nz, 2.0, 1.0, 1.0, 1e7, dtype=dtype, dt=1e-5, scheme="pc",
+                         want_phys=True)
+        job.set_state([w, t])
+        b = fdz.FdzBatch([job])
+        marks = {}
+
+        def on_cfl(i, j):
+            if j.steps in (201, 1001):
+                marks[j.steps] = j.state()
+
+        torch.cuda.synchronize()
+        t_0 = time.perf_counter()
+        fdz.run(b, steps, on_cfl=on_cfl if not timed else None)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t_0
+        st = job.state()
+        b.step(1, write_phys=True)
+        ux, uz, _, tp = job.phys
+        n = nx * nz
+        ke = 0.5 * hip.sum_sq2(ux, uz) / n
+        nu = 1.0 + hip.dot(tp, uz) / n  # conductive flux 1 plus the convective <uz T>
+        kms = None
+        if timed:
+            b.profile(True)
+            b.step(20)
+            ms, cnt = b.profile_read()
+            b.profile(False)
+            kms = [a / max(c, 1) for a, c in zip(ms, cnt)]
+        b.close()
+        return {"state": st, "marks": marks, "seconds": el, "dt": job.dt, "ke": float(ke),
+                "nu": float(nu), "kernel_ms": kms}
+
+    r32 = run("f32", w0, t0, timed=True)
+    r32m = run("f32", w0, t0)
+    r64 = run("f64", w0, t0)
+    w0r = w0.astype(np.complex64).astype(np.complex128)
+    t0r = t0.astype(np.complex64).astype(np.complex128)
+    r64p = run("f64", w0r, t0r)
+
+    def err(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    res = {
+        "workload": "Rayleigh-Benard Ra=1e7 Pr=1, Fourier-x/FD-z 1024x513 (nn=341), "
+                    "AB2/AM2 PC + theta-implicit z diffusion, CFL ticker from dt=1e-5",
+        "steps": steps, "value": nx * nz * steps / r32["seconds"], "unit": "grid-pt-steps/s",
+        "ms_per_step": r32["seconds"] * 1e3 / steps,
+        "fp64_value": nx * nz * steps / r64["seconds"],
+        "kernel_ms_per_launch": dict(zip(["x_transforms", "z_rows"], r32["kernel_ms"])),
+        "final_dt": r32["dt"], "final_dt_fp64": r64["dt"],
+        "rel_err_fp32_vs_fp64": {"omega": err(r32["state"][0], r64["state"][0]),
+                                 "T": err(r32["state"][1], r64["state"][1])},
+        "rel_diff_fp64_from_fp32_rounded_start": {
+            "omega": err(r64p["state"][0], r64["state"][0]),
+            "T": err(r64p["state"][1], r64["state"][1])},
+        "by_step": {
+            str(k): {"fp32_vs_fp64_omega": err(r32m["marks"][k][0], r64["marks"][k][0]),
+                     "fp64_rounded_start_omega": err(r64p["marks"][k][0], r64["marks"][k][0])}
+            for k in sorted(r64["marks"]) if k in r64p["marks"] and k in r32m["marks"]},
+        "kinetic_energy": {"fp32": r32["ke"], "fp64": r64["ke"]},
+        "nusselt": {"fp32": r32["nu"], "fp64": r64["nu"]},
+    }
+    print(f"[bench] rbc: {res}", file=sys.stderr, flush=True)
+    return res
+
+
 def serving_leg(m, args):
     """Serving: seeded Poisson arrivals at 20 jobs/s for ``serving_seconds``
     (vortex pair or resistive tearing, grids from {256..2048}^2, 100-5000
@@ -197,6 +281,8 @@ def main():
     ap.add_argument("--ddc-jobs", type=int, default=64,
                     help="double-diffusive batch leg: number of simulations (0 = skip)")
     ap.add_argument("--ddc-steps", type=int, default=500)
+    ap.add_argument("--rbc-steps", type=int, default=2000,
+                    help="steps of the Rayleigh-Benard FD-z leg (0 skips it)")
     ap.add_argument("--serving-seconds", type=float, default=8.0,
                     help="serving leg: length of the Poisson arrival window (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -282,6 +368,7 @@ def main():
     launch_bytes = kb[names[dom]] * args.batch
     achieved = launch_bytes / (kms32[dom] * 1e-3) / 1e9
     ddc = ddc_leg(m, args) if args.ddc_jobs else None
+    rbc = rbc_leg(m, args) if args.rbc_steps else None
     serving = serving_leg(m, args) if args.serving_seconds > 0 else None
     out = {
         "metric": METRIC,
@@ -314,6 +401,7 @@ def main():
                      "algorithmic_bytes_per_launch": launch_bytes},
         "cpu_baseline": cpu,
         "ddc_batch": ddc,
+        "rbc_fdz": rbc,
         "serving": serving,
     }
     print(json.dumps(out))

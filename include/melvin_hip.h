@@ -217,7 +217,7 @@ int mlv_batch_profile_read(mlv_batch* batch, double* ms, long long* launches);
 typedef struct mlv_fdz_batch mlv_fdz_batch;
 
 typedef struct mlv_fdz_job_desc {
-  int nx, nz;          /* nx a power of two in [8, 4096]; nz in [5, 768] */
+  int nx, nz;          /* nx a power of two in [8, 2048]; nz in [5, 768] */
   double lx, lz;       /* dz = lz / nz (Parameters.py:82) */
   int fd_order;        /* 2 or 4: z differences of advection and explicit
                           diffusion, the x difference, ux = -pddz(psi) */

@@ -87,7 +87,7 @@ int build_fdz_job(const mlv_fdz_job_desc& d, FdzJob<T>& J, int j) {
   };
   std::memset(&J, 0, sizeof(J));
   const int lnx = ilog2_exact(d.nx);
-  if (lnx < 3 || lnx > 12) return bad("nx must be a power of two in [8, 4096]");
+  if (lnx < 3 || lnx > 11) return bad("nx must be a power of two in [8, 2048]");
   if (d.nz < 5 || d.nz > 64 * kFdzMaxK) return bad("nz must be in [5, 768]");
   if (d.fd_order != 2 && d.fd_order != 4) return bad("fd_order must be 2 or 4");
   if (d.ab_order != 2 && d.ab_order != 4) return bad("ab_order must be 2 or 4");

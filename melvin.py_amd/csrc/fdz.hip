@@ -370,7 +370,13 @@ __global__ void __launch_bounds__(64 * kFdzWaves)
 }
 
 // --------------------------------------------------------------------------
-// Column kernel: TC z columns per block, two LDS slots per column.
+// Column kernel: TC z columns per block (TC even), two LDS slots per column.
+//
+// Two real sequences share one complex transform (a + i b).  The rounding
+// of a transform is relative to the whole complex sequence, so the pair
+// must hold values of similar size: omega is ~Ra times T once convection
+// starts, and packing omega with T would bury T under omega's rounding.
+// Each slot therefore pairs the same field at two adjacent z columns.
 // --------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ void herm_pack(cx<T>* __restrict__ z, int k, int N, cx<T> a, cx<T> b) {
@@ -384,6 +390,11 @@ __device__ __forceinline__ void herm_pack(cx<T>* __restrict__ z, int k, int N, c
   }
 }
 
+template <typename T>
+__host__ __device__ constexpr int fdz_tc(int slots) {
+  return slots / 2 > kNT ? kNT : slots / 2;
+}
+
 template <typename T, int LOGN>
 __device__ __forceinline__ void fdz_cols_body(const FdzJob<T>& J, int local, int phase, int wphys,
                                               int step, const cx<T>* __restrict__ tw,
@@ -391,86 +402,122 @@ __device__ __forceinline__ void fdz_cols_body(const FdzJob<T>& J, int local, int
   constexpr int N = 1 << LOGN;
   constexpr int P = fft_points_col<T>(LOGN);
   constexpr int SL = fft_slots<LOGN, kNT, P>();
-  if constexpr (SL < 2) {
+  constexpr int TC = fdz_tc<T>(SL);
+  if constexpr (TC < 2) {
     return;  // rejected on the host
   } else {
-    constexpr int TC = (SL / 2) > kNT ? kNT : SL / 2;
     constexpr int LDSN = fft_ldsn(N);
     const int Z = J.nz, nn = J.nn;
     const int par = (J.par0 + step) & 1;
     const bool corr = J.scheme == MLV_FDZ_PC && phase == 1;
     const int j0 = local * TC;
     const int ncol = min(TC, Z - j0);
+    const int npair = (ncol + 1) / 2;
     const int vs = corr ? 2 : par;
-    const cx<T>* __restrict__ ux = J.vel[vs][0];
-    const cx<T>* __restrict__ uz = J.vel[vs][1];
-    const cx<T>* __restrict__ w = corr ? J.star[0] : J.state[par][0];
-    const cx<T>* __restrict__ t = corr ? J.star[1] : J.state[par][1];
+    // field f of column pair p lives in slot 4 p + f: 0 ux, 1 uz, 2 omega, 3 T
+    const cx<T>* __restrict__ src[4] = {J.vel[vs][0], J.vel[vs][1],
+                                        corr ? J.star[0] : J.state[par][0],
+                                        corr ? J.star[1] : J.state[par][1]};
+    // The x mean of T, Tm(z) = Re T[kx=0], is O(1) (the conductive profile)
+    // while the rest is small: transforming T whole in single precision would
+    // spread Tm's rounding over every mode.  Tm is constant along x, so
+    // F_x(u Tm) = Tm F_x(u) exactly; the transforms carry T - Tm and the
+    // Tm u_s term is added per mode after the forward transform.
+    __shared__ T tmean[kNT];
+    for (int cc = threadIdx.x; cc < ncol; cc += kNT) tmean[cc] = src[3][j0 + cc].x;
+    const int nslot = 4 * npair;
     const int nzero = N - 2 * nn + 1;
-    for (int e = threadIdx.x; e < 2 * ncol * nzero; e += kNT) {
-      const int s = e / nzero, k = nn + e % nzero;
-      lds[s * LDSN + fft_pad(k)] = czero<T>();
+    for (int e = threadIdx.x; e < nslot * nzero; e += kNT) {
+      const int sl = e / nzero, k = nn + e % nzero;
+      lds[sl * LDSN + fft_pad(k)] = czero<T>();
     }
+    // one work item = (field, k, pair): both columns' values of one field;
+    // neighbouring threads read neighbouring column pairs
+    const int nfill = npair * 4 * nn;
     constexpr int U = 4;
-    const int nfill = TC * nn;
 #pragma unroll 1
     for (int e0 = threadIdx.x; e0 < nfill; e0 += U * kNT) {
-      cx<T> a[U], b[U], c[U], d[U];
+      cx<T> a[U], b[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int e = e0 + u * kNT;
-        const int cc = e % TC, k = e / TC;
-        if (e < nfill && cc < ncol) {
-          const size_t idx = (size_t)k * Z + j0 + cc;
-          a[u] = ux[idx];
-          b[u] = uz[idx];
-          c[u] = w[idx];
-          d[u] = t[idx];
+        const int pr = e % npair, k = (e / npair) % nn, f = e / (npair * nn);
+        const int c0 = 2 * pr;
+        a[u] = czero<T>();
+        b[u] = czero<T>();
+        if (e < nfill) {
+          const size_t idx = (size_t)k * Z + j0 + c0;
+          const bool drop = f == 3 && k == 0;  // T without its x mean
+          if (!drop) {
+            a[u] = src[f][idx];
+            if (c0 + 1 < ncol) b[u] = src[f][idx + 1];
+          }
         }
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int e = e0 + u * kNT;
-        const int cc = e % TC, k = e / TC;
-        if (e < nfill && cc < ncol) {
-          herm_pack<T>(lds + (2 * cc) * LDSN, k, N, a[u], b[u]);
-          herm_pack<T>(lds + (2 * cc + 1) * LDSN, k, N, c[u], d[u]);
+        if (e < nfill) {
+          const int pr = e % npair, k = (e / npair) % nn, f = e / (npair * nn);
+          herm_pack<T>(lds + (4 * pr + f) * LDSN, k, N, a[u], b[u]);
         }
       }
     }
     __syncthreads();
-    fft_lds<T, LOGN, +1, kNT, P>(lds, 2 * ncol, tw);
+    fft_lds<T, LOGN, +1, kNT, P>(lds, nslot, tw);
     const bool wp = wphys && !corr;
-    for (int e = threadIdx.x; e < ncol * N; e += kNT) {
-      const int cc = e / N, x = e % N;
-      cx<T>* za = lds + (2 * cc) * LDSN + fft_pad(x);
-      cx<T>* zb = lds + (2 * cc + 1) * LDSN + fft_pad(x);
-      const cx<T> v = *za, s = *zb;  // (ux, uz), (omega, T)
+    for (int e = threadIdx.x; e < npair * N; e += kNT) {
+      const int pr = e / N, x = e % N;
+      cx<T>* z = lds + (4 * pr) * LDSN + fft_pad(x);
+      const cx<T> vx = z[0], vz = z[LDSN], w = z[2 * LDSN], t = z[3 * LDSN];
+      // real parts: column 2 pr, imaginary parts: column 2 pr + 1
       if (wp) {
-        const size_t o = (size_t)x * Z + j0 + cc;
-        if (J.phys[0]) J.phys[0][o] = v.x;
-        if (J.phys[1]) J.phys[1][o] = v.y;
-        if (J.phys[2]) J.phys[2][o] = s.x;
-        if (J.phys[3]) J.phys[3][o] = s.y;
+        const int c0 = 2 * pr;
+        const size_t o = (size_t)x * Z + j0 + c0;
+        const bool two = c0 + 1 < ncol;
+        if (J.phys[0]) { J.phys[0][o] = vx.x; if (two) J.phys[0][o + 1] = vx.y; }
+        if (J.phys[1]) { J.phys[1][o] = vz.x; if (two) J.phys[1][o + 1] = vz.y; }
+        if (J.phys[2]) { J.phys[2][o] = w.x; if (two) J.phys[2][o + 1] = w.y; }
+        if (J.phys[3]) {
+          J.phys[3][o] = t.x + tmean[c0];
+          if (two) J.phys[3][o + 1] = t.y + tmean[c0 + 1];
+        }
       }
-      *za = mk<T>(v.x * s.x, v.y * s.x);
-      *zb = mk<T>(v.x * s.y, v.y * s.y);
+      z[0] = mk<T>(vx.x * w.x, vx.y * w.y);              // ux omega
+      z[LDSN] = mk<T>(vz.x * w.x, vz.y * w.y);           // uz omega
+      z[2 * LDSN] = mk<T>(vx.x * t.x, vx.y * t.y);       // ux T'
+      z[3 * LDSN] = mk<T>(vz.x * t.x, vz.y * t.y);       // uz T'
     }
     __syncthreads();
-    fft_lds<T, LOGN, -1, kNT, P>(lds, 2 * ncol, tw);
+    fft_lds<T, LOGN, -1, kNT, P>(lds, nslot, tw);
     const T hs = (T)(0.5 / (double)N);
-    for (int e = threadIdx.x; e < nfill; e += kNT) {
+    // one work item = (column, k): the four outputs of that column
+    for (int e = threadIdx.x; e < TC * nn; e += kNT) {
       const int cc = e % TC, k = e / TC;
       if (cc >= ncol) continue;
+      const int pr = cc >> 1;
+      const bool hi = cc & 1;
       const size_t idx = (size_t)k * Z + j0 + cc;
+      cx<T> ov[4];
 #pragma unroll
-      for (int fi = 0; fi < 2; ++fi) {
-        const cx<T>* z = lds + (2 * cc + fi) * LDSN;
+      for (int m = 0; m < 4; ++m) {
+        const cx<T>* z = lds + (4 * pr + m) * LDSN;
         const cx<T> y = z[fft_pad(k)];
         const cx<T> ym = z[fft_pad((N - k) & (N - 1))];
-        J.qw[2 * fi][idx] = mk<T>(hs * (y.x + ym.x), hs * (y.y - ym.y));
-        J.qw[2 * fi + 1][idx] = mk<T>(hs * (y.y + ym.y), hs * (ym.x - y.x));
+        // (y + conj ym) / 2 is the first sequence's spectrum, (y - conj ym) / 2i the second's
+        ov[m] = hi ? mk<T>(hs * (y.y + ym.y), hs * (ym.x - y.x))
+                   : mk<T>(hs * (y.x + ym.x), hs * (y.y - ym.y));
       }
+      cx<T> vx = src[0][idx], vz = src[1][idx];
+      if (k == 0) {  // irfft keeps the real part of the kx = 0 term
+        vx.y = T(0);
+        vz.y = T(0);
+      }
+      const T tm = tmean[cc];
+      J.qw[0][idx] = ov[0];
+      J.qw[1][idx] = ov[1];
+      J.qw[2][idx] = mk<T>(ov[2].x + tm * vx.x, ov[2].y + tm * vx.y);
+      J.qw[3][idx] = mk<T>(ov[3].x + tm * vz.x, ov[3].y + tm * vz.y);
     }
   }
 }

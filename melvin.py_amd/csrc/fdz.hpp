@@ -69,11 +69,12 @@ struct FdzLaunch {
   int *prefrow[3][kFdzVariants];
 };
 
-// Blocks of the column kernel per job and its columns per block.
+// Columns per block of the column kernel (two LDS slots each, in pairs of
+// columns); 0 when the LDS slots do not hold two columns.
 template <typename T>
 inline int fdz_cols_per_block(int lognx) {
   const int s = slots_for<T>(lognx) / 2;
-  return s < 1 ? 0 : (s > kNT ? kNT : s);
+  return s < 2 ? 0 : (s > kNT ? kNT : s);
 }
 
 template <typename T>

@@ -175,3 +175,23 @@ def tearing_device(n, seed=0, width=0.1, amp=1e-4, length=1.0, double=False):
     g.manual_seed(int(seed))
     w = amp * (2 * torch.rand((n, n), generator=g, device="cuda", dtype=torch.float64) - 1)
     return _to_spectral_device(w, length, double), _to_spectral_device(j, length, double)
+
+
+def _spectral_fdz(phys):
+    """Fourier-x / FD-z spectrum of a real (nx, nz) field: rfft along x / nx,
+    rows kx < nn (SpectralTransformer.py:63-88)."""
+    nx = phys.shape[0]
+    nn = (nx - 1) // 3
+    return (np.fft.rfft(phys, axis=0) / nx)[:nn].astype(np.complex128)
+
+
+def rbc_conduction(nx, nz, lx=2.0, lz=1.0, seed=0, amp=1e-3):
+    """Rayleigh-Benard start: omega = 0 and the conductive profile
+    T = 1 - z/lz plus seeded uniform noise of amplitude ``amp`` on the
+    z grid of rayleigh_benard_convection.py:22-25 (endpoint=False).
+    Returns (omega, T) Fourier-x / FD-z spectra."""
+    x = np.linspace(0, lx, nx, endpoint=False)
+    z = np.linspace(0, lz, nz, endpoint=False)
+    _, Z = np.meshgrid(x, z, indexing="ij")
+    t = 1.0 - Z / lz + noise((nx, nz), amp, seed)
+    return _spectral_fdz(np.zeros((nx, nz))), _spectral_fdz(t)

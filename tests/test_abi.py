@@ -113,7 +113,8 @@ def test_fdz_argument_errors_are_reported():
     d.hist[0] = d.hist[1] = dummy
     d.work = dummy
     h = ctypes.c_void_p()
-    for field, bad, word in (("nx", 48, b"power of two"), ("nz", 769, b"nz"),
+    for field, bad, word in (("nx", 48, b"power of two"), ("nx", 4096, b"power of two"),
+                             ("nz", 769, b"nz"),
                              ("scheme", 7, b"scheme"), ("fd_order", 3, b"fd_order"),
                              ("ab_order", 3, b"ab_order"), ("dt", 0.0, b"positive")):
         good = getattr(d, field)
