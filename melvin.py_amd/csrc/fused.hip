@@ -24,6 +24,7 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
   cx<T>* __restrict__ dst = J.work + (size_t)op.ch * N * nm;
   cx<T>* __restrict__ psi_out = (op.op == XOP_UX && op.src == 0 && !op.from_new) ? J.psi : nullptr;
   const double fx = J.fx, fz = J.fz, fx2 = J.fx2, fz2 = J.fz2;
+  const float ffx = (float)fx, ffz = (float)fz, ffx2 = (float)fx2, ffz2 = (float)fz2;
   // Each thread owns ITER = P elements; loads are issued U at a time so
   // several are in flight before the first is used.
   constexpr int ITER = (N * TC) / kNT;
@@ -54,18 +55,29 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
           v = f;
         } else {
           const int n = r <= nn ? r : r - rows;
-          double lap = fx2 * (double)(n * n) + fz2 * (double)(m * m);
-          if (r == 0 && m == 0) lap = 1.0;
-          const T rl = (T)(1.0 / lap);
+          T rl, yz, yx;
+          if constexpr (sizeof(T) == 8) {
+            double lap = fx2 * (double)(n * n) + fz2 * (double)(m * m);
+            if (r == 0 && m == 0) lap = 1.0;
+            rl = 1.0 / lap;
+            yz = fz * m;
+            yx = fx * n;
+          } else {
+            // single precision: the symbol and its reciprocal in FP32
+            // (hardware reciprocal, 1 ulp) instead of an FP64 division
+            float lap = ffx2 * (float)(n * n) + ffz2 * (float)(m * m);
+            if (r == 0 && m == 0) lap = 1.0f;
+            rl = __builtin_amdgcn_rcpf(lap);
+            yz = ffz * (float)m;
+            yx = ffx * (float)n;
+          }
           // psi = solve(-f) (utility.py:65, LaplacianSolver.py:64-66)
           const cx<T> ps = mk<T>(-f.x * rl, -f.y * rl);
           if (psi_out) psi_out[idx] = ps;
           if (op.op == XOP_UX) {  // ux = -d(psi)/dz
-            const T y = (T)(fz * m);
-            v = mk<T>(y * ps.y, -(y * ps.x));
+            v = mk<T>(yz * ps.y, -(yz * ps.x));
           } else {  // uz = d(psi)/dx
-            const T y = (T)(fx * n);
-            v = mk<T>(-(y * ps.y), y * ps.x);
+            v = mk<T>(-(yx * ps.y), yx * ps.x);
           }
         }
       }
@@ -130,41 +142,42 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     const int nseq = nrows * NIN;
     const int nslot = (nseq + 1) / 2;
     cx<T>* __restrict__ work = J.work;
-    const int nzero = N - 2 * nm + 1;
-    for (int e = threadIdx.x; e < nslot * nzero; e += kNT) {
-      const int s = e / nzero, k = nm + e % nzero;
-      lds[s * LDSN + fft_pad(k)] = czero<T>();
-    }
-    constexpr int UB = 4;
-    const int nfill = nslot * nm;
+    // One wavefront per slot: lane l handles k = l + 64 i, so no index
+    // needs a division by the runtime row length nm.
+    constexpr int KIT = ((N - 1) / 3 + 63) / 64;  // bound on ceil(nm / 64)
+    constexpr int KB = KIT < 6 ? KIT : 6;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int s = wv; s < nslot; s += kNT / 64) {
+      cx<T>* z = lds + s * LDSN;
+      const int qa = 2 * s, qb = qa + 1;
+      const cx<T>* __restrict__ pa = work + (size_t)(qa % NIN) * chs + (size_t)(x0 + qa / NIN) * nm;
+      const cx<T>* __restrict__ pb =
+          qb < nseq ? work + (size_t)(qb % NIN) * chs + (size_t)(x0 + qb / NIN) * nm : nullptr;
+      // loads of up to KB iterations in flight before any is used
 #pragma unroll 1
-    for (int e0 = threadIdx.x; e0 < nfill; e0 += UB * kNT) {
-      cx<T> av[UB], bv[UB];
+      for (int i0 = 0; i0 < KIT; i0 += KB) {
+        cx<T> av[KB], bv[KB];
 #pragma unroll
-      for (int u = 0; u < UB; ++u) {
-        const int e = e0 + u * kNT;
-        const int s = e / nm, k = e % nm;
-        const int qa = 2 * s, qb = qa + 1;
-        av[u] = e < nfill ? work[(size_t)(qa % NIN) * chs + (size_t)(x0 + qa / NIN) * nm + k]
-                          : czero<T>();
-        bv[u] = (e < nfill && qb < nseq)
-                    ? work[(size_t)(qb % NIN) * chs + (size_t)(x0 + qb / NIN) * nm + k]
-                    : czero<T>();
-      }
+        for (int i = 0; i < KB; ++i) {
+          const int k = lane + 64 * (i0 + i);
+          av[i] = k < nm ? pa[k] : czero<T>();
+          bv[i] = (pb && k < nm) ? pb[k] : czero<T>();
+        }
 #pragma unroll
-      for (int u = 0; u < UB; ++u) {
-        const int e = e0 + u * kNT;
-        if (e >= nfill) continue;
-        const int s = e / nm, k = e % nm;
-        const cx<T> a = av[u], b = bv[u];
-        cx<T>* z = lds + s * LDSN;
-        if (k == 0) {
-          z[0] = mk<T>(a.x, b.x);
-        } else {
-          z[fft_pad(k)] = mk<T>(a.x - b.y, a.y + b.x);
-          z[fft_pad(N - k)] = mk<T>(a.x + b.y, b.x - a.y);
+        for (int i = 0; i < KB; ++i) {
+          const int k = lane + 64 * (i0 + i);
+          if (k < nm) {
+            const cx<T> a = av[i], b = bv[i];
+            if (k == 0) {
+              z[0] = mk<T>(a.x, b.x);
+            } else {
+              z[fft_pad(k)] = mk<T>(a.x - b.y, a.y + b.x);
+              z[fft_pad(N - k)] = mk<T>(a.x + b.y, b.x - a.y);
+            }
+          }
         }
       }
+      for (int k = nm + lane; k <= N - nm; k += 64) z[fft_pad(k)] = czero<T>();
     }
     __syncthreads();
     fft_lds<T, LOGN, +1, kNT, P>(lds, nslot, tw);
@@ -201,15 +214,21 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     }
     __syncthreads();
     fft_lds<T, LOGN, -1, kNT, P>(lds, nrows * NEQ, tw);
-    for (int e = threadIdx.x; e < nrows * NEQ * nm; e += kNT) {
-      const int o = e / nm, k = e % nm;
+    for (int o = wv; o < nrows * NEQ; o += kNT / 64) {
       const int r = o / NEQ, eq = o % NEQ;
       const cx<T>* z = lds + o * LDSN;
-      const cx<T> y = z[fft_pad(k)];
-      const cx<T> ym = z[fft_pad((N - k) & (N - 1))];
-      const size_t off = (size_t)(x0 + r) * nm + k;
-      work[(size_t)pchan(NIN, eq, 0) * chs + off] = mk<T>(T(0.5) * (y.x + ym.x), T(0.5) * (y.y - ym.y));
-      work[(size_t)pchan(NIN, eq, 1) * chs + off] = mk<T>(T(0.5) * (y.y + ym.y), T(0.5) * (ym.x - y.x));
+      cx<T>* __restrict__ ox = work + (size_t)pchan(NIN, eq, 0) * chs + (size_t)(x0 + r) * nm;
+      cx<T>* __restrict__ oz = work + (size_t)pchan(NIN, eq, 1) * chs + (size_t)(x0 + r) * nm;
+#pragma unroll 6
+      for (int i = 0; i < KIT; ++i) {
+        const int k = lane + 64 * i;
+        if (k < nm) {
+          const cx<T> y = z[fft_pad(k)];
+          const cx<T> ym = z[fft_pad((N - k) & (N - 1))];
+          ox[k] = mk<T>(T(0.5) * (y.x + ym.x), T(0.5) * (y.y - ym.y));
+          oz[k] = mk<T>(T(0.5) * (y.y + ym.y), T(0.5) * (ym.x - y.x));
+        }
+      }
     }
   }
 }
