@@ -160,6 +160,10 @@ typedef struct mlv_job_desc {
   double coeff[4];     /* diffusion coefficient per field equation */
   double param[4];     /* DDC: Pr, R0 */
   int nfield;
+  /* The fused batch keeps its spectral arrays kz-major: (nm, 2nn+1), the
+   * element of row r (kx) and column m (kz) at m * (2nn+1) + r, so that a kz
+   * column is contiguous for both x passes.  This applies to state, hist,
+   * comp and psi. */
   void* state[2][4];   /* ping-pong spectral state per field */
   void* hist[4];       /* AB ring per field: ab_order x spectral */
   void* comp[4];       /* F32 only, optional: per-field Kahan carry of the

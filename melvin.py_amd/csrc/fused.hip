@@ -35,21 +35,22 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
     cx<T> fv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
+      // rows fastest: a column of the kz-major state is contiguous
       const int e = threadIdx.x + (i0 + u) * kNT;
-      const int c = e % TC, x = e / TC;
+      const int c = e >> LOGN, x = e & (N - 1);
       const int r = x <= nn ? x : (x >= N - nn ? x - N + rows : -1);
-      fv[u] = (c < ncol && r >= 0) ? src[(size_t)r * nm + c0 + c] : czero<T>();
+      fv[u] = (c < ncol && r >= 0) ? src[(size_t)(c0 + c) * rows + r] : czero<T>();
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
     const int e = threadIdx.x + (i0 + u) * kNT;
-    const int c = e % TC, x = e / TC;
+    const int c = e >> LOGN, x = e & (N - 1);
     cx<T> v = czero<T>();
     if (c < ncol) {
       const int r = x <= nn ? x : (x >= N - nn ? x - N + rows : -1);
       if (r >= 0) {
         const int m = c0 + c;
-        const size_t idx = (size_t)r * nm + m;
+        const size_t idx = (size_t)m * rows + r;
         const cx<T> f = fv[u];
         if (op.op == XOP_ID) {
           v = f;
@@ -399,114 +400,119 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   const cx<T>* __restrict__ sin_ = J.state[par][field];
   cx<T>* __restrict__ sout = J.state[par ^ 1][field];
   cx<T>* __restrict__ hist = J.hist[field];
-  // Epilogue: every thread keeps column cme; its elements are rows
-  // r = threadIdx.x / TC + i * (kNT / TC).  Loads of U2 elements are issued
-  // before any is used.
-  constexpr int RSTEP = kNT / TC;
+  // Epilogue: one wavefront per column, lanes on consecutive rows (the
+  // spectral state is kz-major, so a column is contiguous).  Loads of U2
+  // row groups are issued before any is used.
+  constexpr int KR = (2 * ((N - 1) / 3) + 1 + 63) / 64;  // bound on ceil(rows / 64)
   constexpr int U2 = 4;
-  const int m = c0 + cme;
-  const int r0 = threadIdx.x / TC;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const bool ddc = J.problem == MLV_PROB_DDC;
   cx<T>* __restrict__ carry = sizeof(T) == 4 ? J.comp[field] : nullptr;
   const int prev1 = (cur + q - 1) % q, prev2 = (cur + q - 2) % q, prev3 = (cur + q - 3) % q;
   const float fcoeff = (float)coeff, ffx2 = (float)J.fx2, ffz2 = (float)J.fz2;
   const float fdt = (float)dt, fc2 = (float)c2;
 #pragma unroll 1
-  for (int rb = r0; rb < rows; rb += U2 * RSTEP) {
-    cx<T> vv[U2], h1[U2], h2[U2], h3[U2], lo[U2], e1[U2], e2[U2];
+  for (int cc = wv; cc < ncol; cc += kNT / 64) {
+    const int m = c0 + cc;
+    const size_t cbase = (size_t)m * rows;
+    const cx<T>* __restrict__ dcol = lds + cc * LDSN;
+#pragma unroll 1
+    for (int i0 = 0; i0 < KR; i0 += U2) {
+      cx<T> vv[U2], h1[U2], h2[U2], h3[U2], lo[U2], e1[U2], e2[U2];
 #pragma unroll
-    for (int u = 0; u < U2; ++u) {
-      const int r = rb + u * RSTEP;
-      if (cok && r < rows) {
-        const size_t idx = (size_t)r * nm + m;
-        vv[u] = sin_[idx];
-        h1[u] = hist[(size_t)prev1 * ssz + idx];
-        if (q == 4) {
-          h2[u] = hist[(size_t)prev2 * ssz + idx];
-          h3[u] = hist[(size_t)prev3 * ssz + idx];
+      for (int u = 0; u < U2; ++u) {
+        const int r = lane + 64 * (i0 + u);
+        if (r < rows) {
+          const size_t idx = cbase + r;
+          vv[u] = sin_[idx];
+          h1[u] = hist[(size_t)prev1 * ssz + idx];
+          if (q == 4) {
+            h2[u] = hist[(size_t)prev2 * ssz + idx];
+            h3[u] = hist[(size_t)prev3 * ssz + idx];
+          }
+          if (carry) lo[u] = carry[idx];
+          if (ddc) {
+            if (field == 0) {
+              e1[u] = J.state[par][1][idx];
+              e2[u] = J.state[par][2][idx];
+            } else {
+              e1[u] = J.state[par][0][idx];
+            }
+          }
         }
-        if (carry) lo[u] = carry[idx];
+      }
+#pragma unroll
+      for (int u = 0; u < U2; ++u) {
+        const int r = lane + 64 * (i0 + u);
+        if (r >= rows) continue;
+        const int x = r <= nn ? r : r + N - rows;
+        const int n = r <= nn ? r : r - rows;
+        const size_t idx = cbase + r;
+        const cx<T> D = dcol[fft_pad(x)];
+        cx<T> rhs = mk<T>(D.x * scale, D.y * scale);
         if (ddc) {
+          const double Pr = J.param[0], R0 = J.param[1];
           if (field == 0) {
-            e1[u] = J.state[par][1][idx];
-            e2[u] = J.state[par][2][idx];
+            const T y = (T)(J.fx * n);
+            const cx<T> t = e1[u], sv = e2[u];
+            const T pr = (T)Pr;
+            rhs = cadd<T>(rhs, mk<T>(pr * -(y * sv.y), pr * (y * sv.x)));
+            rhs = csub<T>(rhs, mk<T>(pr * -(y * t.y), pr * (y * t.x)));
           } else {
-            e1[u] = J.state[par][0][idx];
+            double lap = J.fx2 * (double)(n * n) + J.fz2 * (double)(m * m);
+            if (r == 0 && m == 0) lap = 1.0;
+            const T rl = (T)(1.0 / lap);
+            const cx<T> w = e1[u];
+            const cx<T> ps = mk<T>(-w.x * rl, -w.y * rl);
+            const T y = (T)(J.fx * n);
+            cx<T> uz = mk<T>(-(y * ps.y), y * ps.x);
+            if (field == 2) {
+              const T ir0 = (T)(1.0 / R0);
+              uz = mk<T>(uz.x * ir0, uz.y * ir0);
+            }
+            rhs = csub<T>(rhs, uz);
           }
         }
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < U2; ++u) {
-      const int r = rb + u * RSTEP;
-      if (!cok || r >= rows) continue;
-      const int x = r <= nn ? r : r + N - rows;
-      const int n = r <= nn ? r : r - rows;
-      const size_t idx = (size_t)r * nm + m;
-      const cx<T> D = lds[cme * LDSN + fft_pad(x)];
-      cx<T> rhs = mk<T>(D.x * scale, D.y * scale);
-      if (ddc) {
-        const double Pr = J.param[0], R0 = J.param[1];
-        if (field == 0) {
-          const T y = (T)(J.fx * n);
-          const cx<T> t = e1[u], sv = e2[u];
-          const T pr = (T)Pr;
-          rhs = cadd<T>(rhs, mk<T>(pr * -(y * sv.y), pr * (y * sv.x)));
-          rhs = csub<T>(rhs, mk<T>(pr * -(y * t.y), pr * (y * t.x)));
+        hist[(size_t)cur * ssz + idx] = rhs;
+        T abx, aby;
+        if (q == 2) {
+          abx = T(3) * rhs.x - h1[u].x;
+          aby = T(3) * rhs.y - h1[u].y;
         } else {
-          double lap = J.fx2 * (double)(n * n) + J.fz2 * (double)(m * m);
-          if (r == 0 && m == 0) lap = 1.0;
-          const T rl = (T)(1.0 / lap);
-          const cx<T> w = e1[u];
-          const cx<T> ps = mk<T>(-w.x * rl, -w.y * rl);
-          const T y = (T)(J.fx * n);
-          cx<T> uz = mk<T>(-(y * ps.y), y * ps.x);
-          if (field == 2) {
-            const T ir0 = (T)(1.0 / R0);
-            uz = mk<T>(uz.x * ir0, uz.y * ir0);
+          abx = T(55) * rhs.x - T(59) * h1[u].x + T(37) * h2[u].x - T(9) * h3[u].x;
+          aby = T(55) * rhs.y - T(59) * h1[u].y + T(37) * h2[u].y - T(9) * h3[u].y;
+        }
+        const cx<T> v = vv[u];
+        cx<T> nv;
+        const bool masked = ddc && field != 0 && m == 0;
+        if constexpr (sizeof(T) == 8) {
+          // (1 + (1-a) dt L) v + AB, divided by (1 - a dt L)  (Integrator.py:58-63)
+          const double lin = coeff * (J.fx2 * (double)(n * n) + J.fz2 * (double)(m * m));
+          const T num = (T)(1.0 + c1 * lin);
+          const T rden = (T)(1.0 / (1.0 - c2 * lin));
+          nv = mk<T>((num * v.x + (T)hab * abx) * rden, (num * v.y + (T)hab * aby) * rden);
+        } else {
+          // The same update written as an increment, v + (dt L v + AB)/(1 - a dt L),
+          // so the rounding falls on the small change, not on (1 + (1-a) dt L);
+          // the factors therefore only need single precision.  With a carry
+          // array the addition is Kahan-compensated: increments below half an
+          // ulp of v are kept instead of being dropped every step.
+          const float lin = fcoeff * (ffx2 * (float)(n * n) + ffz2 * (float)(m * m));
+          const T a = fdt * lin;
+          const T rden = __builtin_amdgcn_rcpf(1.0f - fc2 * lin);
+          const T ix = (a * v.x + (T)hab * abx) * rden, iy = (a * v.y + (T)hab * aby) * rden;
+          if (carry) {
+            const T yx = ix - lo[u].x, yy = iy - lo[u].y;
+            const T tx = v.x + yx, ty = v.y + yy;
+            carry[idx] = masked ? czero<T>() : mk<T>((tx - v.x) - yx, (ty - v.y) - yy);
+            nv = mk<T>(tx, ty);
+          } else {
+            nv = mk<T>(v.x + ix, v.y + iy);
           }
-          rhs = csub<T>(rhs, uz);
         }
+        if (masked) nv = czero<T>();
+        sout[idx] = nv;
       }
-      hist[(size_t)cur * ssz + idx] = rhs;
-      T abx, aby;
-      if (q == 2) {
-        abx = T(3) * rhs.x - h1[u].x;
-        aby = T(3) * rhs.y - h1[u].y;
-      } else {
-        abx = T(55) * rhs.x - T(59) * h1[u].x + T(37) * h2[u].x - T(9) * h3[u].x;
-        aby = T(55) * rhs.y - T(59) * h1[u].y + T(37) * h2[u].y - T(9) * h3[u].y;
-      }
-      const cx<T> v = vv[u];
-      cx<T> nv;
-      const bool masked = ddc && field != 0 && m == 0;
-      if constexpr (sizeof(T) == 8) {
-        // (1 + (1-a) dt L) v + AB, divided by (1 - a dt L)  (Integrator.py:58-63)
-        const double lin = coeff * (J.fx2 * (double)(n * n) + J.fz2 * (double)(m * m));
-        const T num = (T)(1.0 + c1 * lin);
-        const T rden = (T)(1.0 / (1.0 - c2 * lin));
-        nv = mk<T>((num * v.x + (T)hab * abx) * rden, (num * v.y + (T)hab * aby) * rden);
-      } else {
-        // The same update written as an increment, v + (dt L v + AB)/(1 - a dt L),
-        // so the rounding falls on the small change, not on (1 + (1-a) dt L);
-        // the factors therefore only need single precision.  With a carry
-        // array the addition is Kahan-compensated: increments below half an
-        // ulp of v are kept instead of being dropped every step.
-        const float lin = fcoeff * (ffx2 * (float)(n * n) + ffz2 * (float)(m * m));
-        const T a = fdt * lin;
-        const T rden = __builtin_amdgcn_rcpf(1.0f - fc2 * lin);
-        const T ix = (a * v.x + (T)hab * abx) * rden, iy = (a * v.y + (T)hab * aby) * rden;
-        if (carry) {
-          const T yx = ix - lo[u].x, yy = iy - lo[u].y;
-          const T tx = v.x + yx, ty = v.y + yy;
-          carry[idx] = masked ? czero<T>() : mk<T>((tx - v.x) - yx, (ty - v.y) - yy);
-          nv = mk<T>(tx, ty);
-        } else {
-          nv = mk<T>(v.x + ix, v.y + iy);
-        }
-      }
-      if (masked) nv = czero<T>();
-      sout[idx] = nv;
     }
   }
 }

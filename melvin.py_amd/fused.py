@@ -71,7 +71,9 @@ class Job:
         self.nm = (self.nz - 1) // 3
         self.rows = 2 * self.nn + 1
         dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
-        shape = (self.rows, self.nm)
+        # device arrays are kz-major, (nm, rows): a kz column is contiguous,
+        # which is how both x passes read and write the state
+        shape = (self.nm, self.rows)
         nf = self.nfield
         self.buf = [[torch.zeros(shape, dtype=self.cplx, device=dev) for _ in range(nf)]
                     for _ in range(2)]
@@ -96,6 +98,7 @@ class Job:
     # -- host access --------------------------------------------------------
     @property
     def spectral_shape(self):
+        """Shape of the spectra the host sees (the reference layout)."""
         return (self.rows, self.nm)
 
     def hbm_bytes(self):
@@ -119,26 +122,34 @@ class Job:
             if tuple(t.shape) != self.spectral_shape:
                 raise ValueError(f"field {i} has shape {tuple(t.shape)}, expected "
                                  f"{self.spectral_shape}")
-            t = t.to(self.buf[0][0].device, torch.complex128)
+            t = t.to(self.buf[0][0].device, torch.complex128).transpose(0, 1)
             hi = t.to(self.cplx)
             self.buf[self.parity][i].copy_(hi)
             if self.carry is not None:
                 self.carry[i].copy_((hi.to(torch.complex128) - t).to(self.cplx))
 
     def current(self):
-        """Device tensors of the current spectral state."""
-        return self.buf[self.parity]
+        """Device tensors of the current spectral state, as (rows, nm) views
+        of the kz-major buffers."""
+        return [b.transpose(0, 1) for b in self.buf[self.parity]]
 
     def state(self):
         """Current spectral state as complex128 host arrays (including the
         single-precision carry when there is one: value = buf - carry)."""
         out = []
-        for i, b in enumerate(self.current()):
+        for i, b in enumerate(self.buf[self.parity]):
             v = b.detach().cpu().numpy().astype(np.complex128)
             if self.carry is not None:
                 v = v - self.carry[i].detach().cpu().numpy().astype(np.complex128)
-            out.append(v)
+            out.append(np.ascontiguousarray(v.T))
         return out
+
+    def psi_spectral(self):
+        """Streamfunction of the last step (needs ``want_psi``), as a
+        complex128 host array in the reference layout."""
+        if self.psi is None:
+            raise RuntimeError("job was created without psi output")
+        return np.ascontiguousarray(self.psi.detach().cpu().numpy().astype(np.complex128).T)
 
     def physical(self):
         """Physical fields written by the last step run with write_phys."""

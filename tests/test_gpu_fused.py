@@ -24,7 +24,7 @@ def _golden_jobs(mlv, dtype="f64"):
     j = _job(mlv, "vorticity", params, dtype, coeff=[0.05], want_psi=True, want_phys=True)
     j.set_state([g["w0_spec"]])
     out.append((j, len(g["dts"]), lambda j, g=g: [(j.state()[0], g["w_spec"]),
-                                                  (j.psi.cpu().numpy(), g["psi_spec"])]))
+                                                  (j.psi_spectral(), g["psi_spec"])]))
     g, params, _ = golden("kh_ab4")
     j = _job(mlv, "vorticity", params, dtype, coeff=[1e-3, 1e-3], want_phys=True)
     j.set_state([g["w0_spec"], g["c0_spec"]])
@@ -92,7 +92,7 @@ def test_fused_physical_outputs(mlv):
     psi, oux, ouz, _, _ = O.velocity_from_vorticity(p, g["w0_spec"])
     # uz here is mostly high-kx noise, the least favourable case for rounding
     for got, want in ((w, O.to_physical(p, g["w0_spec"])), (c, O.to_physical(p, g["c0_spec"])),
-                      (ux, oux), (uz, ouz), (j.psi.cpu().numpy(), psi)):
+                      (ux, oux), (uz, ouz), (j.psi_spectral(), psi)):
         assert rel(got, want) < 1e-12
 
 
@@ -131,7 +131,7 @@ def test_config1_taylor_green_256_fp64(mlv):
     O.run(p, st, lambda p, s: O.step_vorticity(p, s, 1e-2), 1000)
     assert j.dt == st["dt"] == 1e-3
     assert rel(j.state()[0], st["w"]) <= 1e-12
-    assert rel(j.psi.cpu().numpy(), st["psi"]) <= 1e-12
+    assert rel(j.psi_spectral(), st["psi"]) <= 1e-12
     amp = j.state()[0][1, 1] / w0[1, 1]
     assert abs(amp - np.exp(-2 * 1e-2 * 1.0)) < 1e-6
 
