@@ -703,10 +703,12 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
   const size_t lds_max = max_lds_per_block(b->device);
   const size_t jsz = dtype == MLV_F64 ? sizeof(JobDev<double>) : sizeof(JobDev<float>);
   std::vector<char> host(jsz * njobs);
-  std::vector<int> pA[2], pB[2][BV_COUNT], pC[2];
+  std::vector<int> pA[2], pB[2][BV_COUNT], pC[2], pCn[2];
+  size_t ldsCn = 0;
   for (int s = 0; s < 2; ++s) {
     pA[s].assign(njobs + 1, 0);
     pC[s].assign(njobs + 1, 0);
+    pCn[s].assign(njobs + 1, 0);
     for (int v = 0; v < BV_COUNT; ++v) pB[s][v].assign(njobs + 1, 0);
   }
   BatchLaunch& bl = b->bl;
@@ -744,18 +746,21 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
         bvar[s] = J.st[s].bvar;
       }
     }
-    const int slx = dtype == MLV_F64 ? fused_cols_host<double>(lognx) : fused_cols_host<float>(lognx);
-    const int slz = dtype == MLV_F64 ? slots_row<double>(lognz) : slots_row<float>(lognz);
-    const size_t ldsx = dtype == MLV_F64 ? fft_lds_bytes<double>(lognx) : fft_lds_bytes<float>(lognx);
-    const size_t ldsz =
-        dtype == MLV_F64 ? fft_lds_bytes_row<double>(lognz) : fft_lds_bytes_row<float>(lognz);
-    if (ldsx > lds_max || ldsz > lds_max || slx < 1) {
+    const bool f64 = dtype == MLV_F64;
+    // x passes: narrow (n) and wide (w) points policies, see fused_points
+    const int sln = f64 ? fused_cols_host<double>(lognx, 0) : fused_cols_host<float>(lognx, 0);
+    const int slw = f64 ? fused_cols_host<double>(lognx, 1) : fused_cols_host<float>(lognx, 1);
+    const size_t ldsn = f64 ? fused_lds_bytes<double>(lognx, 0) : fused_lds_bytes<float>(lognx, 0);
+    const size_t ldsw = f64 ? fused_lds_bytes<double>(lognx, 1) : fused_lds_bytes<float>(lognx, 1);
+    const int slz = f64 ? slots_row<double>(lognz) : slots_row<float>(lognz);
+    const size_t ldsz = f64 ? fft_lds_bytes_row<double>(lognz) : fft_lds_bytes_row<float>(lognz);
+    if (ldsn > lds_max || ldsw > lds_max || ldsz > lds_max || sln < 1 || slw < 1) {
       rc = fail(MLV_ERR_UNSUPPORTED, "job " + std::to_string(j) + ": grid too large for LDS");
       break;
     }
     if (nstage > b->nstage) b->nstage = nstage;
     for (int s = 0; s < 2; ++s) {
-      int a = 0, c = 0, bb = 0, bv = 0;
+      int a = 0, c = 0, cn = 0, bb = 0, bv = 0;
       if (s < nstage) {
         const int nin_v = bvar_nin(bvar[s]), neq_v = bvar_neq(bvar[s]);
         if (!bfits(nin_v, neq_v, slz)) {
@@ -764,19 +769,37 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
           break;
         }
         const int rb = brows(nin_v, neq_v, slz);
-        a = nA[s] * ((nm + slx - 1) / slx);
-        c = neq[s] * ((nm + slx - 1) / slx);
+        a = nA[s] * ((nm + sln - 1) / sln);
+        c = neq[s] * ((nm + slw - 1) / slw);
+        cn = neq[s] * ((nm + sln - 1) / sln);
         bb = (nx + rb - 1) / rb;
         bv = bvar[s];
         bl.ldsB[bv] = std::max(bl.ldsB[bv], ldsz);
       }
       pA[s][j + 1] = pA[s][j] + a;
       pC[s][j + 1] = pC[s][j] + c;
+      pCn[s][j + 1] = pCn[s][j] + cn;
       for (int v = 0; v < BV_COUNT; ++v) pB[s][v][j + 1] = pB[s][v][j] + (v == bv ? bb : 0);
     }
     if (rc) break;
-    bl.ldsA = std::max(bl.ldsA, ldsx);
-    bl.ldsC = std::max(bl.ldsC, ldsx);
+    bl.ldsA = std::max(bl.ldsA, ldsn);
+    bl.ldsC = std::max(bl.ldsC, ldsw);
+    ldsCn = std::max(ldsCn, ldsn);
+  }
+  if (!rc) {
+    // Kernel C runs wide (more columns per block) only when its grid covers
+    // the device's block slots twice over at two blocks per CU; a smaller
+    // grid is better spread with narrow blocks (measured on MI355X: KH
+    // 1024^2, 8 jobs narrow -10% time, 32 jobs wide -7%).
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    bl.wideC = pC[0][njobs] >= 4 * cus ? 1 : 0;
+    if (!bl.wideC) {
+      for (int s = 0; s < 2; ++s) pC[s] = pCn[s];
+      bl.ldsC = ldsCn;
+    }
   }
   if (rc) {
     delete b;

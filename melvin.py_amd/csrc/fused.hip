@@ -11,7 +11,7 @@ template <typename T, int LOGN>
 __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S, int local,
                                            int par, const cx<T>* __restrict__ tw, cx<T>* lds) {
   constexpr int N = 1 << LOGN;
-  constexpr int P = fft_points_col<T>(LOGN);
+  constexpr int P = fused_points<T>(LOGN, 0);
   constexpr int TC = fused_cols<LOGN, P>();
   constexpr int LDSN = fft_ldsn(N);
   const int nn = J.nn, nm = J.nm, rows = J.rows;
@@ -269,11 +269,11 @@ __device__ __forceinline__ T modk(const cx<T>* __restrict__ tw, int k, int logn,
   return (T(8) * s1 - s2) / (T)(6.0 * h);
 }
 
-template <typename T, int LOGN>
+template <typename T, int LOGN, int WIDE>
 __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S, int local, int cur,
                                            int par, const cx<T>* __restrict__ tw, cx<T>* lds) {
   constexpr int N = 1 << LOGN;
-  constexpr int P = fft_points_col<T>(LOGN);
+  constexpr int P = fused_points<T>(LOGN, WIDE);
   constexpr int TC = fused_cols<LOGN, P>();
   constexpr int LDSN = fft_ldsn(N);
   const int nn = J.nn, nm = J.nm, rows = J.rows;
@@ -517,7 +517,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   }
 }
 
-template <typename T>
+template <typename T, int WIDE>
 __global__ void __launch_bounds__(kNT) k_stepC(const JobDev<T>* __restrict__ jobs,
                                                const int* __restrict__ pref, int njobs, int stage,
                                                int step, const cx<T>* __restrict__ tw) {
@@ -533,7 +533,7 @@ __global__ void __launch_bounds__(kNT) k_stepC(const JobDev<T>* __restrict__ job
   switch (J.lognx) {
 #define X(L)                                                            \
   case L:                                                               \
-    stepC_body<T, L>(J, J.st[stage], local, cur, par, tw, lds);         \
+    stepC_body<T, L, WIDE>(J, J.st[stage], local, cur, par, tw, lds);   \
     break;
     X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 #undef X
@@ -576,9 +576,15 @@ hipError_t fused_stage(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, 
   launchB<T, BV_MHD2>(jobs, bl, stage, wphys, tw, s);
   if (ev) (void)hipEventRecord(ev[2], s);
   if (bl.nC[stage] > 0) {
-    MLV_SET_LDS_ONCE(k_stepC<T>, bl.ldsC);
-    hipLaunchKernelGGL(k_stepC<T>, dim3(xcd_grid(bl.nC[stage])), dim3(kNT), bl.ldsC, s, jobs,
-                       (const int*)bl.prefC[stage], bl.njobs, stage, step, tw);
+    if (bl.wideC) {
+      MLV_SET_LDS_ONCE((k_stepC<T, 1>), bl.ldsC);
+      hipLaunchKernelGGL((k_stepC<T, 1>), dim3(xcd_grid(bl.nC[stage])), dim3(kNT), bl.ldsC, s,
+                         jobs, (const int*)bl.prefC[stage], bl.njobs, stage, step, tw);
+    } else {
+      MLV_SET_LDS_ONCE((k_stepC<T, 0>), bl.ldsC);
+      hipLaunchKernelGGL((k_stepC<T, 0>), dim3(xcd_grid(bl.nC[stage])), dim3(kNT), bl.ldsC, s,
+                         jobs, (const int*)bl.prefC[stage], bl.njobs, stage, step, tw);
+    }
   }
   if (ev) (void)hipEventRecord(ev[3], s);
   return hipGetLastError();
@@ -591,7 +597,8 @@ hipError_t fused_prepare(const BatchLaunch& bl) {
   size_t b = 0;
   for (int v = 0; v < BV_COUNT; ++v) b = bl.ldsB[v] > b ? bl.ldsB[v] : b;
   if (bl.ldsA) MLV_SET_LDS_ONCE(k_stepA<T>, bl.ldsA);
-  if (bl.ldsC) MLV_SET_LDS_ONCE(k_stepC<T>, bl.ldsC);
+  if (bl.ldsC && bl.wideC) MLV_SET_LDS_ONCE((k_stepC<T, 1>), bl.ldsC);
+  if (bl.ldsC && !bl.wideC) MLV_SET_LDS_ONCE((k_stepC<T, 0>), bl.ldsC);
   if (bl.ldsB[BV_VORT1]) MLV_SET_LDS_ONCE((k_stepB<T, BV_VORT1>), bl.ldsB[BV_VORT1]);
   if (bl.ldsB[BV_VORT2]) MLV_SET_LDS_ONCE((k_stepB<T, BV_VORT2>), bl.ldsB[BV_VORT2]);
   if (bl.ldsB[BV_VORT3]) MLV_SET_LDS_ONCE((k_stepB<T, BV_VORT3>), bl.ldsB[BV_VORT3]);

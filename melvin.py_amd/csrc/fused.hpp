@@ -120,10 +120,25 @@ template <int LOGN, int P>
 __host__ __device__ constexpr int fused_cols() {
   return fft_slots<LOGN, kNT, P>() < kNT ? fft_slots<LOGN, kNT, P>() : kNT;
 }
+// Points per thread of the x passes.  wide = 1 is the column-pass policy
+// of fft_points_col (32 points from N = 1024 in single precision: more
+// columns per block); wide = 0 keeps 16 points up to N = 1024 (half the LDS
+// per block, so twice the blocks per CU).  The x-inverse always runs narrow;
+// the x-forward/update kernel runs wide when its grid fills the device
+// several times over and narrow otherwise.
 template <typename T>
-inline int fused_cols_host(int logn) {
-  const int s = slots_for<T>(logn);
+__host__ __device__ constexpr int fused_points(int logn, int wide) {
+  return wide ? fft_points_col<T>(logn) : (logn >= 11 ? 32 : 16);
+}
+template <typename T>
+inline int fused_cols_host(int logn, int wide) {
+  const int s = (kNT * fused_points<T>(logn, wide)) >> logn;
   return s < kNT ? s : kNT;
+}
+template <typename T>
+inline size_t fused_lds_bytes(int logn, int wide) {
+  const int s = (kNT * fused_points<T>(logn, wide)) >> logn;
+  return (size_t)s * fft_ldsn(1 << logn) * sizeof(cx<T>);
 }
 
 // Rows per z-pass block: as many as fit the LDS slots, at most 8.
@@ -149,6 +164,7 @@ __host__ __device__ constexpr int pchan(int base, int e, int k) { return base + 
 
 struct BatchLaunch {
   int njobs;
+  int wideC;                           // points policy of kernel C (fused_points)
   int nA[2], nB[2][BV_COUNT], nC[2];  // total blocks per launch
   size_t ldsA, ldsB[BV_COUNT], ldsC;
   int* prefA[2];                       // device prefix arrays (njobs + 1)
