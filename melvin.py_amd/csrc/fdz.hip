@@ -568,17 +568,26 @@ __global__ void k_fdz_factors(double2* __restrict__ fac, int nn, int nz, int K, 
   }
   for (int e = 0; e < K * 64; ++e) F[e] = make_double2(0.0, 1.0);
   F[0] = make_double2(0.0, 1.0);
-  double cprev = 0.0;
-  for (int j = 1; j < nz; ++j) {
-    const bool last = j == nz - 1;
-    const double l = last ? 0.0 : lo;
-    const double dj = last ? 1.0 : dg;
-    const double up = last ? 0.0 : lo;
-    const double rd = 1.0 / (dj - l * cprev);
-    const double c = up * rd;
+  // The interior recurrence has constant coefficients; once it maps a
+  // value to itself in floating point every later interior factor is that
+  // same value, so the rest of the interior is filled without recomputing
+  // (identical results, far fewer dependent divisions for diffusion).
+  double cprev = 0.0, rprev = 0.0;
+  int j = 1;
+  for (; j < nz - 1; ++j) {
+    const double rd = 1.0 / (dg - lo * cprev);
+    const double c = lo * rd;
     F[(j % K) * 64 + j / K] = make_double2(c, rd);
+    const bool fixed = c == cprev && rd == rprev;
     cprev = c;
+    rprev = rd;
+    if (fixed) {
+      for (++j; j < nz - 1; ++j) F[(j % K) * 64 + j / K] = make_double2(c, rd);
+      break;
+    }
   }
+  // last row: identity (lower 0, diagonal 1), cp = 0
+  F[((nz - 1) % K) * 64 + (nz - 1) / K] = make_double2(0.0, 1.0);
 }
 
 template <typename T>

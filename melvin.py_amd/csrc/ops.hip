@@ -303,6 +303,30 @@ __device__ __forceinline__ double red_combine(double a, double b, int op) {
   return op == MLV_RED_MAX ? fmax(a, b) : a + b;
 }
 
+// Pairwise tree over the block's (sum, compensation) pairs in LDS: a
+// NaN-propagating max, or Neumaier-compensated addition of the pairs.
+// Leaves the result in slot 0.
+__device__ __forceinline__ void red_tree(double* sh_s, double* sh_c, int op) {
+  for (int w = kEW / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      const int o = threadIdx.x + w;
+      if (op == MLV_RED_MAX) {
+        const double a = sh_s[threadIdx.x], b = sh_s[o];
+        sh_s[threadIdx.x] = (a != a || b != b) ? NAN : fmax(a, b);
+      } else {
+        KSum k;
+        k.s = sh_s[threadIdx.x];
+        k.c = sh_c[threadIdx.x];
+        k.add(sh_s[o]);
+        k.add(sh_c[o]);
+        sh_s[threadIdx.x] = k.s;
+        sh_c[threadIdx.x] = k.c;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kEW) k_reduce_part(const T* __restrict__ a, const T* __restrict__ b,
                                                      size_t n, int op, double* __restrict__ part) {
@@ -332,47 +356,40 @@ __global__ void __launch_bounds__(kEW) k_reduce_part(const T* __restrict__ a, co
     sh_c[threadIdx.x] = acc.c;
   }
   __syncthreads();
+  red_tree(sh_s, sh_c, op);
   if (threadIdx.x == 0) {
-    if (op == MLV_RED_MAX) {
-      double m = -INFINITY;
-      bool anynan = false;
-      for (int t = 0; t < kEW; ++t) {
-        if (sh_s[t] != sh_s[t]) anynan = true;
-        m = fmax(m, sh_s[t]);
-      }
-      part[2 * blockIdx.x] = anynan ? NAN : m;
-      part[2 * blockIdx.x + 1] = 0.0;
-    } else {
-      KSum k;
-      for (int t = 0; t < kEW; ++t) {
-        k.add(sh_s[t]);
-        k.add(sh_c[t]);
-      }
-      part[2 * blockIdx.x] = k.s;
-      part[2 * blockIdx.x + 1] = k.c;
-    }
+    part[2 * blockIdx.x] = sh_s[0];
+    part[2 * blockIdx.x + 1] = sh_c[0];
   }
 }
 
-__global__ void __launch_bounds__(64) k_reduce_final(const double* __restrict__ part, int nblk, int op,
-                                                     double scale, double* __restrict__ out) {
-  if (threadIdx.x != 0) return;
+__global__ void __launch_bounds__(kEW) k_reduce_final(const double* __restrict__ part, int nblk,
+                                                      int op, double scale,
+                                                      double* __restrict__ out) {
+  __shared__ double sh_s[kEW], sh_c[kEW];
+  // each thread folds a strided subset of the partials, then a tree
   if (op == MLV_RED_MAX) {
     double m = -INFINITY;
-    bool anynan = false;
-    for (int t = 0; t < nblk; ++t) {
-      if (part[2 * t] != part[2 * t]) anynan = true;
-      m = fmax(m, part[2 * t]);
+    bool nan = false;
+    for (int t = threadIdx.x; t < nblk; t += kEW) {
+      const double v = part[2 * t];
+      if (v != v) nan = true;
+      m = fmax(m, v);
     }
-    out[0] = anynan ? NAN : m;
+    sh_s[threadIdx.x] = nan ? NAN : m;
+    sh_c[threadIdx.x] = 0.0;
   } else {
     KSum k;
-    for (int t = 0; t < nblk; ++t) {
+    for (int t = threadIdx.x; t < nblk; t += kEW) {
       k.add(part[2 * t]);
       k.add(part[2 * t + 1]);
     }
-    out[0] = k.value() * scale;
+    sh_s[threadIdx.x] = k.s;
+    sh_c[threadIdx.x] = k.c;
   }
+  __syncthreads();
+  red_tree(sh_s, sh_c, op);
+  if (threadIdx.x == 0) out[0] = op == MLV_RED_MAX ? sh_s[0] : (sh_s[0] + sh_c[0]) * scale;
 }
 
 // --------------------------------------------------------------------------
@@ -425,7 +442,7 @@ hipError_t launch_reduce(const void* a, const void* b, size_t n, int op, double 
   if (g < 1) g = 1;
   hipLaunchKernelGGL(k_reduce_part<T>, dim3((unsigned)g), dim3(kEW), 0, s, (const T*)a,
                      (const T*)b, n, op, part);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, s, (const double*)part, (int)g, op,
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kEW), 0, s, (const double*)part, (int)g, op,
                      scale, out);
   return hipGetLastError();
 }
