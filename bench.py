@@ -62,15 +62,11 @@ def kernel_bytes(nx, nz, nf, q, cbytes, carry=False):
     return {"x_inverse": a, "z_pass": b, "x_forward_update": c}
 
 
-KERNEL_SYMBOLS = {"x_inverse": "mlv::k_stepA<float>", "z_pass": "mlv::k_stepB<float, 1>",
-                  "x_forward_update": "mlv::k_stepC<float>"}
-
-
 def measured_traffic(kernel, batch):
-    """HBM bytes per launch of ``kernel`` read by rocprofv3's FETCH_SIZE and
-    WRITE_SIZE counters (calibrated against a plain copy of known size) for
-    this workload at this batch size, from the committed counter summary;
-    None when no summary covers the configuration."""
+    """HBM bytes per launch of ``kernel`` (by role) read by rocprofv3's
+    FETCH_SIZE and WRITE_SIZE counters, calibrated against a plain copy of
+    known size, for this workload at this batch size, from the committed
+    counter summary; None when no summary covers the configuration."""
     path = os.path.join(ROOT, "profiles", f"kh{NX}_batch{batch}_hbm_traffic.md")
     try:
         text = open(path).read()
@@ -78,7 +74,7 @@ def measured_traffic(kernel, batch):
         return None
     block = text.split("```json", 1)[-1].split("```", 1)[0]
     try:
-        return json.loads(block)["bytes_per_launch"].get(KERNEL_SYMBOLS[kernel])
+        return json.loads(block)["bytes_per_launch"].get(kernel)
     except (ValueError, KeyError):
         return None
 
