@@ -13,7 +13,7 @@ The directory is ``melvin.py_amd``; it is imported under the package name
 ``melvin_amd`` (see ``__graft_entry__.load_package``).
 """
 from . import hip
-from . import fdz, fused, initial, serving, utility
+from . import checkpoint, fdz, fused, initial, serving, utility
 from .ArrayFactory import ArrayFactory
 from .BasisFunctions import BasisFunctions
 from .DataTransferer import DataTransferer
