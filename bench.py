@@ -51,14 +51,15 @@ def kernel_bytes(nx, nz, nf, q, cbytes, carry=False):
     simulation with nf advected fields: the minimum reads and writes of its
     pass (each operand once).  x_forward_update reads Px, Pz, the state and
     q-1 history slots, writes the state and one history slot, and with the
-    single-precision carry also reads and writes that."""
+    single-precision residual also reads and writes that (bfloat16 pairs,
+    half a complex value each)."""
     nn, nm = (nx - 1) // 3, (nz - 1) // 3
     S = (2 * nn + 1) * nm      # compact spectral
     I = nx * nm                # x-pass intermediate
     nch = nf + 2               # fields + ux + uz
     a = (nf * S + nch * I) * cbytes
     b = (nch * I + 2 * nf * I) * cbytes
-    c = (2 * nf * I + nf * S * (1 + (q - 1) + 2 + (2 if carry else 0))) * cbytes
+    c = (2 * nf * I + nf * S * (1 + (q - 1) + 2 + (1 if carry else 0))) * cbytes
     return {"x_inverse": a, "z_pass": b, "x_forward_update": c}
 
 

@@ -166,8 +166,10 @@ typedef struct mlv_job_desc {
    * comp and psi. */
   void* state[2][4];   /* ping-pong spectral state per field */
   void* hist[4];       /* AB ring per field: ab_order x spectral */
-  void* comp[4];       /* F32 only, optional: per-field Kahan carry of the
-                          state (spectral, zero-initialised), or NULL */
+  void* comp[4];       /* F32 only, optional: per-field Kahan residual of the
+                          state (value = state - residual) as bfloat16 (re, im)
+                          pairs, 4 bytes per spectral element,
+                          zero-initialised; or NULL */
   void* work;          /* mlv_job_work_bytes() bytes */
   void* psi;           /* optional spectral psi of the step, or NULL */
   void* phys[8];       /* optional physical outputs (problem order), or NULL */

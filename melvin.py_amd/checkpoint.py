@@ -32,7 +32,10 @@ def _arrays(job):
     if isinstance(job, fused.Job):
         state = [b.transpose(0, 1) for b in cur]
         hist = [h.transpose(1, 2) for h in job.hist]
-        carry = [c.transpose(0, 1) for c in job.carry] if job.carry is not None else None
+        # bfloat16 residual pairs, kept as their 16-bit patterns (numpy has
+        # no bfloat16)
+        carry = ([c.view(torch.int16).transpose(0, 1) for c in job.carry]
+                 if job.carry is not None else None)
     else:
         state, hist, carry = list(cur), list(job.hist), job.carry
     return state, hist, carry

@@ -624,7 +624,7 @@ int build_job(const mlv_job_desc& d, JobDev<T>& J, int j) {
   for (int b = 0; b < 2; ++b)
     for (int f = 0; f < nf; ++f) J.state[b][f] = (cx<T>*)d.state[b][f];
   for (int f = 0; f < nf; ++f) J.hist[f] = (cx<T>*)d.hist[f];
-  for (int f = 0; f < nf; ++f) J.comp[f] = sizeof(T) == 4 ? (cx<T>*)d.comp[f] : nullptr;
+  for (int f = 0; f < nf; ++f) J.comp[f] = sizeof(T) == 4 ? (uint32_t*)d.comp[f] : nullptr;
   J.work = (cx<T>*)d.work;
   J.psi = (cx<T>*)d.psi;
   for (int i = 0; i < kMaxPhys; ++i) J.phys[i] = (T*)d.phys[i];

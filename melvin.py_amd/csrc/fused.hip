@@ -407,7 +407,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   constexpr int U2 = 4;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const bool ddc = J.problem == MLV_PROB_DDC;
-  cx<T>* __restrict__ carry = sizeof(T) == 4 ? J.comp[field] : nullptr;
+  uint32_t* __restrict__ carry = sizeof(T) == 4 ? J.comp[field] : nullptr;
   const int prev1 = (cur + q - 1) % q, prev2 = (cur + q - 2) % q, prev3 = (cur + q - 3) % q;
   const float fcoeff = (float)coeff, ffx2 = (float)J.fx2, ffz2 = (float)J.fz2;
   const float fdt = (float)dt, fc2 = (float)c2;
@@ -430,7 +430,10 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
             h2[u] = hist[(size_t)prev2 * ssz + idx];
             h3[u] = hist[(size_t)prev3 * ssz + idx];
           }
-          if (carry) lo[u] = carry[idx];
+          if (carry) {
+            const float2 b = bf2_load(carry[idx]);
+            lo[u] = mk<T>((T)b.x, (T)b.y);
+          }
           if (ddc) {
             if (field == 0) {
               e1[u] = J.state[par][1][idx];
@@ -504,7 +507,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
           if (carry) {
             const T yx = ix - lo[u].x, yy = iy - lo[u].y;
             const T tx = v.x + yx, ty = v.y + yy;
-            carry[idx] = masked ? czero<T>() : mk<T>((tx - v.x) - yx, (ty - v.y) - yy);
+            carry[idx] = masked ? 0u : bf2_pack((float)((tx - v.x) - yx), (float)((ty - v.y) - yy));
             nv = mk<T>(tx, ty);
           } else {
             nv = mk<T>(v.x + ix, v.y + iy);

@@ -70,12 +70,27 @@ struct JobDev {
   int nfield;
   cx<T>* state[2][kMaxFields];
   cx<T>* hist[kMaxFields];
-  cx<T>* comp[kMaxFields];   // optional Kahan carry of the state (single precision)
+  // single precision, optional: the Kahan residual of the state (value =
+  // state - residual) as bfloat16 (re, im) pairs packed in 32 bits; it only
+  // has to resolve a fraction of one FP32 ulp, so 8 bits of it suffice
+  uint32_t* comp[kMaxFields];
   cx<T>* work;
   cx<T>* psi;
   T* phys[kMaxPhys];
   StageDev st[2];
 };
+
+// bfloat16 pair <-> float2 (round to nearest even; the residuals are finite)
+__device__ __forceinline__ float2 bf2_load(uint32_t v) {
+  return make_float2(__uint_as_float(v << 16), __uint_as_float(v & 0xffff0000u));
+}
+__device__ __forceinline__ uint32_t bf16_bits(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+__device__ __forceinline__ uint32_t bf2_pack(float re, float im) {
+  return bf16_bits(re) | (bf16_bits(im) << 16);
+}
 
 // Per-variant channel/product description used by kernel B.
 template <int V> struct BProg;

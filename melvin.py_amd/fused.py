@@ -79,9 +79,11 @@ class Job:
                     for _ in range(2)]
         self.hist = [torch.zeros((self.ab_order,) + shape, dtype=self.cplx, device=dev)
                      for _ in range(nf)]
-        # single precision: Kahan carry of the state update (see the x-forward
-        # kernel); the value held is buf - carry
-        self.carry = ([torch.zeros(shape, dtype=self.cplx, device=dev) for _ in range(nf)]
+        # single precision: Kahan residual of the state update (see the
+        # x-forward kernel), value held = buf - carry; stored as bfloat16
+        # (re, im) pairs: it only resolves a fraction of one FP32 ulp
+        self.carry = ([torch.zeros(shape + (2,), dtype=torch.bfloat16, device=dev)
+                       for _ in range(nf)]
                       if (compensated and not self.double) else None)
         nbytes = ctypes.c_size_t()
         _lib.call("mlv_job_work_bytes", PROBLEMS[problem], nf, self.nx, self.nz,
@@ -126,7 +128,8 @@ class Job:
             hi = t.to(self.cplx)
             self.buf[self.parity][i].copy_(hi)
             if self.carry is not None:
-                self.carry[i].copy_((hi.to(torch.complex128) - t).to(self.cplx))
+                r = torch.view_as_real(hi.to(torch.complex128) - t)
+                self.carry[i].copy_(r.to(torch.bfloat16))
 
     def current(self):
         """Device tensors of the current spectral state, as (rows, nm) views
@@ -140,7 +143,8 @@ class Job:
         for i, b in enumerate(self.buf[self.parity]):
             v = b.detach().cpu().numpy().astype(np.complex128)
             if self.carry is not None:
-                v = v - self.carry[i].detach().cpu().numpy().astype(np.complex128)
+                r = self.carry[i].detach().to(torch.float64).cpu().numpy()
+                v = v - (r[..., 0] + 1j * r[..., 1])
             out.append(np.ascontiguousarray(v.T))
         return out
 

@@ -98,8 +98,8 @@ def footprint(req, dtype="f32"):
     work = ctypes.c_size_t()
     _lib.call("mlv_job_work_bytes", _lib.PROB_VORTICITY if nf == 1 else _lib.PROB_MHD, nf, n, n,
               _lib.F32 if dtype == "f32" else _lib.F64, ctypes.byref(work))
-    carry = nf * S if dtype == "f32" else 0
-    return nf * (2 * S + 2 * S) + carry + work.value   # state x2, AB2 ring, carry, work
+    carry = nf * S // 2 if dtype == "f32" else 0   # bfloat16 residual pairs
+    return nf * (2 * S + 2 * S) + carry + work.value   # state x2, AB2 ring, residual, work
 
 
 class Server:
