@@ -175,6 +175,8 @@ typedef struct mlv_job_desc {
   void* phys[8];       /* optional physical outputs (problem order), or NULL */
   int curr;            /* ring slot the next step writes */
   int parity;          /* state buffer holding the current state */
+  double t;            /* simulated time so far (advanced on the device) */
+  long long loops;     /* loops so far (advanced on the device) */
 } mlv_job_desc;
 
 int mlv_job_work_bytes(int problem, int nfield, int nx, int nz, int dtype, size_t* bytes);
@@ -190,6 +192,25 @@ int mlv_batch_step(mlv_batch* batch, int nsteps, int write_phys, void* stream);
 int mlv_batch_graph_step(mlv_batch* batch, int nsteps, void* stream);
 /* Set one job's dt for the following steps (stream-ordered). */
 int mlv_batch_set_dt(mlv_batch* batch, int job, double dt, void* stream);
+/* nsteps loops with Simulation.end_loop's CFL ticker on the device
+ * (Simulation.py:199-209, Ticker.py:14-25, Integrator.py:35-44): after the
+ * loops whose 0-based count is a multiple of cadence, each job's maxima of
+ * its physical ux, uz (FP64, NaN-propagating, as xp.max) give
+ * cfl_dt = min(dx / max ux, dz / max uz); a breach or NaN raises the job's
+ * breach flag, else dt shrinks by 0.9 until dt <= cutoff * cfl_dt.  No host
+ * synchronisation; graph = 1 replays a captured graph (nsteps and the step
+ * count multiples of 4 and of cadence).  Each check also appends a tracker
+ * record (loop, t, dt, kinetic energy as utility.py:42-59, max ux, max uz,
+ * next dt). */
+int mlv_batch_run(mlv_batch* batch, int nsteps, int cadence, double cutoff, int graph,
+                  void* stream);
+/* One job's device-side dt, t, loop count and CFL breach flag
+ * (synchronises the device). */
+int mlv_batch_job_time(mlv_batch* batch, int job, double* dt, double* t, long long* loops,
+                       int* breach);
+/* Up to max_records tracker records of one job, oldest first, 7 doubles
+ * each (see mlv_batch_run); the last 4096 are kept (synchronises). */
+int mlv_batch_tracker(mlv_batch* batch, int job, double* out, int max_records, int* n_records);
 /* Steps completed since creation (host count). */
 long long mlv_batch_steps(const mlv_batch* batch);
 /* Kernel timing (no reference counterpart; measurement only): while enabled,

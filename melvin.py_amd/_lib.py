@@ -37,6 +37,7 @@ EXPORTS = (
     "mlv_reduce", "mlv_combine", "mlv_velocity", "mlv_job_work_bytes", "mlv_batch_create",
     "mlv_batch_destroy", "mlv_batch_step", "mlv_batch_graph_step", "mlv_batch_set_dt",
     "mlv_batch_steps", "mlv_batch_profile", "mlv_batch_profile_read",
+    "mlv_batch_run", "mlv_batch_job_time", "mlv_batch_tracker",
     "mlv_fdz_work_bytes", "mlv_fdz_batch_create", "mlv_fdz_batch_destroy",
     "mlv_fdz_batch_prime", "mlv_fdz_batch_step", "mlv_fdz_batch_graph_step",
     "mlv_fdz_batch_set_dt", "mlv_fdz_batch_steps", "mlv_fdz_batch_profile",
@@ -72,6 +73,8 @@ class JobDesc(ctypes.Structure):
         ("phys", ctypes.c_void_p * 8),
         ("curr", ctypes.c_int),
         ("parity", ctypes.c_int),
+        ("t", ctypes.c_double),
+        ("loops", ctypes.c_longlong),
     ]
 
 
@@ -133,6 +136,10 @@ def _declare(lib):
         "mlv_batch_steps": ([vp], ctypes.c_longlong),
         "mlv_batch_profile": ([vp, i], i),
         "mlv_batch_profile_read": ([vp, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_longlong)], i),
+        "mlv_batch_run": ([vp, i, i, d, i, vp], i),
+        "mlv_batch_job_time": ([vp, i, ctypes.POINTER(d), ctypes.POINTER(d),
+                                ctypes.POINTER(ctypes.c_longlong), ip], i),
+        "mlv_batch_tracker": ([vp, i, ctypes.POINTER(d), i, ip], i),
         "mlv_fdz_work_bytes": ([i, i, i, ctypes.POINTER(sz)], i),
         "mlv_fdz_batch_create": ([pp, i, ctypes.POINTER(FdzJobDesc), i], i),
         "mlv_fdz_batch_destroy": ([vp], i),

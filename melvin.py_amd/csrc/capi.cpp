@@ -516,11 +516,16 @@ struct BatchHost {
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   int graph_steps = 0;
+  int graph_cadence = 0;
+  double graph_cutoff = 0;
   bool prof = false;
   std::vector<hipEvent_t> events;
   size_t used = 0;
   double ms[3] = {0, 0, 0};
   long long launches[3] = {0, 0, 0};
+  std::vector<char> host;          // host copy of the job table
+  std::vector<double*> diag;       // per job diagnostics buffer
+  std::vector<int> nbB;
 };
 
 hipEvent_t* prof_slots(BatchHost* b) {
@@ -627,6 +632,8 @@ int build_job(const mlv_job_desc& d, JobDev<T>& J, int j) {
   for (int f = 0; f < nf; ++f) J.comp[f] = sizeof(T) == 4 ? (uint32_t*)d.comp[f] : nullptr;
   J.work = (cx<T>*)d.work;
   J.psi = (cx<T>*)d.psi;
+  J.t = d.t;
+  J.loops = d.loops;
   for (int i = 0; i < kMaxPhys; ++i) J.phys[i] = (T*)d.phys[i];
   for (int s = 0; s < 2; ++s)
     for (int i = 0; i < kMaxPhys; ++i) J.st[s].phys_ch[i] = -1;
@@ -776,6 +783,7 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
         bv = bvar[s];
         bl.ldsB[bv] = std::max(bl.ldsB[bv], ldsz);
       }
+      if (s == 0) b->nbB.push_back(bb);
       pA[s][j + 1] = pA[s][j] + a;
       pC[s][j + 1] = pC[s][j] + c;
       pCn[s][j + 1] = pCn[s][j] + cn;
@@ -828,9 +836,28 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
       rc = upload_pref(pB[s][v], &bl.prefB[s][v]);
     }
   }
+  for (int j = 0; j < njobs && !rc; ++j) {
+    double* dg = nullptr;
+    const size_t bytes = diag_doubles(b->nbB[j]) * sizeof(double);
+    rc = dalloc(bytes, (void**)&dg);
+    if (!rc) {
+      hipError_t e = hipMemset(dg, 0, bytes);
+      if (e != hipSuccess) rc = hip_fail(e, "clear diagnostics");
+    }
+    if (rc) break;
+    b->diag.push_back(dg);
+    if (dtype == MLV_F64) {
+      reinterpret_cast<JobDev<double>*>(host.data())[j].diag = dg;
+      reinterpret_cast<JobDev<double>*>(host.data())[j].nbB = b->nbB[j];
+    } else {
+      reinterpret_cast<JobDev<float>*>(host.data())[j].diag = dg;
+      reinterpret_cast<JobDev<float>*>(host.data())[j].nbB = b->nbB[j];
+    }
+  }
   if (!rc) {
     hipError_t e = hipMemcpy(b->jobs_dev, host.data(), host.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) rc = hip_fail(e, "upload jobs");
+    b->host = host;
   }
   if (!rc) {
     hipError_t e = dtype == MLV_F64 ? fused_prepare<double>(bl) : fused_prepare<float>(bl);
@@ -884,7 +911,7 @@ int mlv_batch_profile_read(mlv_batch* bp, double* ms, long long* launches) {
 }
 
 static hipError_t enqueue_steps(BatchHost* b, int nsteps, int write_phys, long long first,
-                                hipStream_t s, bool timed);
+                                hipStream_t s, bool timed, int cadence = 0, double cutoff = 0.5);
 
 int mlv_batch_step(mlv_batch* bp, int nsteps, int write_phys, void* stream) {
   if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
@@ -896,32 +923,46 @@ int mlv_batch_step(mlv_batch* bp, int nsteps, int write_phys, void* stream) {
   return MLV_OK;
 }
 
+// Steps first .. first+nsteps-1 (counted from the batch's creation).  With
+// cadence > 0 the steps whose count is a multiple of cadence (loops 1,
+// 1+cadence, ... of Simulation.end_loop's CFL ticker) also produce the
+// diagnostics partials and are followed by the on-device CFL update.
 static hipError_t enqueue_steps(BatchHost* b, int nsteps, int write_phys, long long first,
-                                hipStream_t s, bool timed) {
+                                hipStream_t s, bool timed, int cadence, double cutoff) {
   for (int k = 0; k < nsteps; ++k) {
-    const int wp = (write_phys && k == nsteps - 1) ? 1 : 0;
+    const int wp = (write_phys && k == nsteps - 1) ? kWritePhys : 0;
+    const bool dg = cadence > 0 && ((first + k) % cadence) == 0;
+    const int flags0 = wp | (dg ? kDiag : 0);
     const int step = (int)((first + k) & 3);
     for (int st = 0; st < b->nstage; ++st) {
       hipEvent_t* ev = timed ? prof_slots(b) : nullptr;
       hipError_t e =
           b->dtype == MLV_F64
-              ? fused_stage<double>((const JobDev<double>*)b->jobs_dev, b->bl, st, st == 0 ? wp : 0,
-                                    step, (const cx<double>*)b->tw, s, ev)
-              : fused_stage<float>((const JobDev<float>*)b->jobs_dev, b->bl, st, st == 0 ? wp : 0,
-                                   step, (const cx<float>*)b->tw, s, ev);
+              ? fused_stage<double>((const JobDev<double>*)b->jobs_dev, b->bl, st,
+                                    st == 0 ? flags0 : 0, step, (const cx<double>*)b->tw, s, ev)
+              : fused_stage<float>((const JobDev<float>*)b->jobs_dev, b->bl, st,
+                                   st == 0 ? flags0 : 0, step, (const cx<float>*)b->tw, s, ev);
+      if (e != hipSuccess) return e;
+    }
+    if (dg) {
+      hipError_t e = b->dtype == MLV_F64
+                         ? fused_diag<double>((JobDev<double>*)b->jobs_dev, b->njobs, cutoff, s)
+                         : fused_diag<float>((JobDev<float>*)b->jobs_dev, b->njobs, cutoff, s);
       if (e != hipSuccess) return e;
     }
   }
   return hipSuccess;
 }
 
-int mlv_batch_graph_step(mlv_batch* bp, int nsteps, void* stream) {
-  if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
-  BatchHost* b = reinterpret_cast<BatchHost*>(bp);
+static int graph_run(BatchHost* b, int nsteps, int cadence, double cutoff, void* stream) {
   if (nsteps <= 0 || (nsteps & 3)) return fail(MLV_ERR_ARG, "graph steps must be a positive multiple of 4");
   if (b->steps & 3)
     return fail(MLV_ERR_ARG, "graph replay needs the batch's step count at a multiple of 4");
-  if (!b->graph_exec || b->graph_steps != nsteps) {
+  if (cadence > 0 && ((nsteps % cadence) || (b->steps % cadence)))
+    return fail(MLV_ERR_ARG, "graph runs with CFL checks need nsteps and the step count at "
+                             "multiples of the cadence");
+  if (!b->graph_exec || b->graph_steps != nsteps || b->graph_cadence != cadence ||
+      b->graph_cutoff != cutoff) {
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->graph_exec = nullptr;
@@ -929,7 +970,7 @@ int mlv_batch_graph_step(mlv_batch* bp, int nsteps, void* stream) {
     if (!b->cap_stream)
       HIP_TRY(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking), "capture stream");
     HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal), "begin capture");
-    hipError_t e = enqueue_steps(b, nsteps, 0, 0, b->cap_stream, false);
+    hipError_t e = enqueue_steps(b, nsteps, 0, 0, b->cap_stream, false, cadence, cutoff);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(b->cap_stream, &g);
     if (e != hipSuccess) return hip_fail(e, "capture");
@@ -937,9 +978,89 @@ int mlv_batch_graph_step(mlv_batch* bp, int nsteps, void* stream) {
     b->graph = g;
     HIP_TRY(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0), "instantiate");
     b->graph_steps = nsteps;
+    b->graph_cadence = cadence;
+    b->graph_cutoff = cutoff;
   }
   HIP_TRY(hipGraphLaunch(b->graph_exec, (hipStream_t)stream), "graph launch");
   b->steps += nsteps;
+  return MLV_OK;
+}
+
+int mlv_batch_graph_step(mlv_batch* bp, int nsteps, void* stream) {
+  if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
+  return graph_run(reinterpret_cast<BatchHost*>(bp), nsteps, 0, 0.0, stream);
+}
+
+int mlv_batch_run(mlv_batch* bp, int nsteps, int cadence, double cutoff, int graph, void* stream) {
+  if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
+  if (nsteps < 0 || cadence < 0 || !(cutoff > 0)) return fail(MLV_ERR_ARG, "bad run arguments");
+  BatchHost* b = reinterpret_cast<BatchHost*>(bp);
+  if (graph) return graph_run(b, nsteps, cadence, cutoff, stream);
+  hipError_t e =
+      enqueue_steps(b, nsteps, 0, b->steps, (hipStream_t)stream, b->prof, cadence, cutoff);
+  if (e != hipSuccess) return hip_fail(e, "fused run");
+  b->steps += nsteps;
+  return MLV_OK;
+}
+
+}  // extern "C"
+
+template <typename T>
+static int read_job(BatchHost* b, int job, double* dt, double* t, long long* loops) {
+  JobDev<T> J;
+  HIP_TRY(hipMemcpy(&J, (const char*)b->jobs_dev + job * sizeof(JobDev<T>), sizeof(J),
+                    hipMemcpyDeviceToHost),
+          "read job");
+  if (dt) *dt = J.dt;
+  if (t) *t = J.t;
+  if (loops) *loops = J.loops;
+  return MLV_OK;
+}
+
+extern "C" {
+
+int mlv_batch_job_time(mlv_batch* bp, int job, double* dt, double* t, long long* loops,
+                       int* breach) {
+  if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
+  BatchHost* b = reinterpret_cast<BatchHost*>(bp);
+  if (job < 0 || job >= b->njobs) return fail(MLV_ERR_ARG, "job index out of range");
+  HIP_TRY(hipDeviceSynchronize(), "synchronize");
+  int rc = b->dtype == MLV_F64 ? read_job<double>(b, job, dt, t, loops)
+                               : read_job<float>(b, job, dt, t, loops);
+  if (rc) return rc;
+  if (breach) {
+    double h[2];
+    HIP_TRY(hipMemcpy(h, b->diag[job] + (size_t)kDiagPart * b->nbB[job], sizeof(h),
+                      hipMemcpyDeviceToHost),
+            "read diagnostics");
+    *breach = h[1] != 0.0;
+  }
+  return MLV_OK;
+}
+
+int mlv_batch_tracker(mlv_batch* bp, int job, double* out, int max_records, int* nrec) {
+  if (!bp || !nrec) return fail(MLV_ERR_ARG, "NULL argument");
+  BatchHost* b = reinterpret_cast<BatchHost*>(bp);
+  if (job < 0 || job >= b->njobs) return fail(MLV_ERR_ARG, "job index out of range");
+  HIP_TRY(hipDeviceSynchronize(), "synchronize");
+  const double* base = b->diag[job] + (size_t)kDiagPart * b->nbB[job];
+  double h[4];
+  HIP_TRY(hipMemcpy(h, base, sizeof(h), hipMemcpyDeviceToHost), "read diagnostics");
+  const long long total = (long long)h[0];
+  const long long kept = total < kDiagCap ? total : kDiagCap;
+  const long long n = kept < max_records ? kept : max_records;
+  // oldest kept record first
+  std::vector<double> all((size_t)kDiagRec * kDiagCap);
+  if (n > 0 && out) {
+    HIP_TRY(hipMemcpy(all.data(), base + 4, all.size() * sizeof(double), hipMemcpyDeviceToHost),
+            "read tracker");
+    for (long long i = 0; i < n; ++i) {
+      const long long r = (total - kept + i) % kDiagCap;
+      std::memcpy(out + (size_t)kDiagRec * i, all.data() + (size_t)kDiagRec * r,
+                  kDiagRec * sizeof(double));
+    }
+  }
+  *nrec = (int)n;
   return MLV_OK;
 }
 

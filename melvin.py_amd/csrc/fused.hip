@@ -4,6 +4,43 @@
 
 namespace mlv {
 
+// NaN-propagating maximum (np.max semantics; fmax would drop a NaN).
+template <typename T>
+__device__ __forceinline__ T nan_max(T a, T b) {
+  return (a != a || b != b) ? T(NAN) : (a > b ? a : b);
+}
+
+// Block reduction of the z pass's diagnostics partials; thread 0 writes
+// (max ux, max uz, sum(ux^2 + uz^2)) to out.
+template <typename T>
+__device__ void block_diag(T mux, T muz, double ke, double* out) {
+  __shared__ double red[kNT / 64][3];
+  double a = (double)mux, b = (double)muz;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a = nan_max(a, __shfl_xor(a, off, 64));
+    b = nan_max(b, __shfl_xor(b, off, 64));
+    ke += __shfl_xor(ke, off, 64);
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[wv][0] = a;
+    red[wv][1] = b;
+    red[wv][2] = ke;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kNT / 64; ++w) {
+      a = nan_max(a, red[w][0]);
+      b = nan_max(b, red[w][1]);
+      ke += red[w][2];
+    }
+    out[0] = a;
+    out[1] = b;
+    out[2] = ke;
+  }
+}
+
 // --------------------------------------------------------------------------
 // Kernel A: spectral prologue + inverse x pass
 // --------------------------------------------------------------------------
@@ -125,7 +162,7 @@ __global__ void __launch_bounds__(kNT) k_stepA(const JobDev<T>* __restrict__ job
 // --------------------------------------------------------------------------
 template <typename T, int LOGN, int V>
 __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S, int local,
-                                           int wphys, const cx<T>* __restrict__ tw, cx<T>* lds) {
+                                           int flags, const cx<T>* __restrict__ tw, cx<T>* lds) {
   using Pg = BProg<V>;
   constexpr int NIN = Pg::NIN, NEQ = Pg::NEQ;
   constexpr int N = 1 << LOGN;
@@ -182,6 +219,9 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     }
     __syncthreads();
     fft_lds<T, LOGN, +1, kNT, P>(lds, nslot, tw);
+    const bool diag = flags & kDiag;
+    T mux = -INFINITY, muz = -INFINITY;  // NaN-propagating maxima, as np.max
+    double ke = 0.0;
     for (int n = threadIdx.x; n < N; n += kNT) {
       T v[RB * NIN];
 #pragma unroll
@@ -193,7 +233,18 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
           v[q] = T(0);
         }
       }
-      if (wphys) {
+      if (diag) {
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+          if (r < nrows) {
+            const T ux = v[r * NIN + Pg::UX], uz = v[r * NIN + Pg::UZ];
+            mux = nan_max(mux, ux);
+            muz = nan_max(muz, uz);
+            ke += (double)uz * (double)uz + (double)ux * (double)ux;
+          }
+        }
+      }
+      if (flags & kWritePhys) {
 #pragma unroll
         for (int q = 0; q < RB * NIN; ++q) {
           if (q < nseq) {
@@ -213,6 +264,7 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
         }
       }
     }
+    if (diag) block_diag<T>(mux, muz, ke, J.diag + (size_t)kDiagPart * local);
     __syncthreads();
     fft_lds<T, LOGN, -1, kNT, P>(lds, nrows * NEQ, tw);
     for (int o = wv; o < nrows * NEQ; o += kNT / 64) {
@@ -237,7 +289,7 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
 template <typename T, int V>
 __global__ void __launch_bounds__(kNT) k_stepB(const JobDev<T>* __restrict__ jobs,
                                                const int* __restrict__ pref, int njobs, int stage,
-                                               int wphys, const cx<T>* __restrict__ tw) {
+                                               int flags, const cx<T>* __restrict__ tw) {
   extern __shared__ char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int lb = xcd_logical(pref[njobs]);
@@ -248,7 +300,7 @@ __global__ void __launch_bounds__(kNT) k_stepB(const JobDev<T>* __restrict__ job
   switch (J.lognz) {
 #define X(L)                                                      \
   case L:                                                         \
-    stepB_body<T, L, V>(J, J.st[stage], local, wphys, tw, lds);   \
+    stepB_body<T, L, V>(J, J.st[stage], local, flags, tw, lds);   \
     break;
     X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 #undef X
@@ -533,6 +585,13 @@ __global__ void __launch_bounds__(kNT) k_stepC(const JobDev<T>* __restrict__ job
   const int local = lb - pref[j];
   const int par = (J.par0 + step) & 1;
   const int cur = ring(J.curr0, step, J.q);
+  if (stage == J.nstage - 1 && local == 0 && threadIdx.x == 0) {
+    // t += dt after each loop (Simulation.py:199-209); nothing else in this
+    // launch reads these two fields
+    JobDev<T>* Jw = const_cast<JobDev<T>*>(jobs) + j;
+    Jw->t += J.dt;
+    Jw->loops += 1;
+  }
   switch (J.lognx) {
 #define X(L)                                                            \
   case L:                                                               \
@@ -550,20 +609,80 @@ __global__ void k_set_dt(JobDev<T>* jobs, int j, double dt) {
   if (threadIdx.x == 0 && blockIdx.x == 0) jobs[j].dt = dt;
 }
 
+// One block per job: reduce the z pass's partials, append a tracker record
+// (loop, t, dt, kinetic energy 0.5 sum(ux^2+uz^2)/(nx nz) as utility.py:
+// 42-59, max ux, max uz, next dt) and apply the CFL rule of Integrator.py:
+// 35-44: cfl_dt = min(dx / max ux, dz / max uz) with Python's min (a NaN in
+// the second operand is ignored, in the first it is returned); a breach or a
+// NaN sets the breach flag and leaves dt, else dt shrinks by 0.9 until
+// dt <= cutoff * cfl_dt.
+template <typename T>
+__global__ void __launch_bounds__(kNT) k_diag(JobDev<T>* jobs, double cutoff) {
+  JobDev<T>& J = jobs[blockIdx.x];
+  __shared__ double red[kNT / 64][3];
+  double a = -INFINITY, b = -INFINITY, ke = 0.0;
+  for (int i = threadIdx.x; i < J.nbB; i += kNT) {
+    const double* p = J.diag + (size_t)kDiagPart * i;
+    a = nan_max(a, p[0]);
+    b = nan_max(b, p[1]);
+    ke += p[2];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a = nan_max(a, __shfl_xor(a, off, 64));
+    b = nan_max(b, __shfl_xor(b, off, 64));
+    ke += __shfl_xor(ke, off, 64);
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[wv][0] = a;
+    red[wv][1] = b;
+    red[wv][2] = ke;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int w = 1; w < kNT / 64; ++w) {
+    a = nan_max(a, red[w][0]);
+    b = nan_max(b, red[w][1]);
+    ke += red[w][2];
+  }
+  const double c1 = J.dx / a, c2 = J.dz / b;
+  const double cfl = c2 < c1 ? c2 : c1;
+  double dt = J.dt;
+  const double used = dt;
+  double* hdr = J.diag + (size_t)kDiagPart * J.nbB;
+  if (dt > cfl || cfl != cfl) {
+    hdr[1] = 1.0;  // breach
+  } else {
+    while (dt > cutoff * cfl) dt = dt * 0.9;
+    J.dt = dt;
+  }
+  const long long nrec = (long long)hdr[0];
+  double* rec = hdr + 4 + (size_t)kDiagRec * (nrec % kDiagCap);
+  rec[0] = (double)J.loops;
+  rec[1] = J.t;
+  rec[2] = used;
+  rec[3] = 0.5 * ke / ((double)J.nx * (double)J.nz);
+  rec[4] = a;
+  rec[5] = b;
+  rec[6] = J.dt;
+  hdr[0] = (double)(nrec + 1);
+}
+
 // --------------------------------------------------------------------------
 // Host side
 // --------------------------------------------------------------------------
 template <typename T, int V>
-static void launchB(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int wphys,
+static void launchB(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int flags,
                     const cx<T>* tw, hipStream_t s) {
   if (bl.nB[stage][V] == 0) return;
   MLV_SET_LDS_ONCE((k_stepB<T, V>), bl.ldsB[V]);
   hipLaunchKernelGGL((k_stepB<T, V>), dim3(xcd_grid(bl.nB[stage][V])), dim3(kNT), bl.ldsB[V], s, jobs,
-                     (const int*)bl.prefB[stage][V], bl.njobs, stage, wphys, tw);
+                     (const int*)bl.prefB[stage][V], bl.njobs, stage, flags, tw);
 }
 
 template <typename T>
-hipError_t fused_stage(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int wphys,
+hipError_t fused_stage(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int flags,
                        int step, const cx<T>* tw, hipStream_t s, hipEvent_t* ev) {
   if (ev) (void)hipEventRecord(ev[0], s);
   if (bl.nA[stage] > 0) {
@@ -572,11 +691,11 @@ hipError_t fused_stage(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, 
                        (const int*)bl.prefA[stage], bl.njobs, stage, step, tw);
   }
   if (ev) (void)hipEventRecord(ev[1], s);
-  launchB<T, BV_VORT1>(jobs, bl, stage, wphys, tw, s);
-  launchB<T, BV_VORT2>(jobs, bl, stage, wphys, tw, s);
-  launchB<T, BV_VORT3>(jobs, bl, stage, wphys, tw, s);
-  launchB<T, BV_MHD1>(jobs, bl, stage, wphys, tw, s);
-  launchB<T, BV_MHD2>(jobs, bl, stage, wphys, tw, s);
+  launchB<T, BV_VORT1>(jobs, bl, stage, flags, tw, s);
+  launchB<T, BV_VORT2>(jobs, bl, stage, flags, tw, s);
+  launchB<T, BV_VORT3>(jobs, bl, stage, flags, tw, s);
+  launchB<T, BV_MHD1>(jobs, bl, stage, flags, tw, s);
+  launchB<T, BV_MHD2>(jobs, bl, stage, flags, tw, s);
   if (ev) (void)hipEventRecord(ev[2], s);
   if (bl.nC[stage] > 0) {
     if (bl.wideC) {
@@ -619,6 +738,13 @@ hipError_t fused_set_dt(JobDev<T>* jobs, int job, double dt, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t fused_diag(JobDev<T>* jobs, int njobs, double cutoff, hipStream_t s) {
+  hipLaunchKernelGGL(k_diag<T>, dim3(njobs), dim3(kNT), 0, s, jobs, cutoff);
+  return hipGetLastError();
+}
+template hipError_t fused_diag<float>(JobDev<float>*, int, double, hipStream_t);
+template hipError_t fused_diag<double>(JobDev<double>*, int, double, hipStream_t);
 template hipError_t fused_set_dt<float>(JobDev<float>*, int, double, hipStream_t);
 template hipError_t fused_set_dt<double>(JobDev<double>*, int, double, hipStream_t);
 template hipError_t fused_stage<float>(const JobDev<float>*, const BatchLaunch&, int, int, int,

@@ -78,7 +78,22 @@ struct JobDev {
   cx<T>* psi;
   T* phys[kMaxPhys];
   StageDev st[2];
+  // device-side time keeping and diagnostics (see k_diag): the update kernel
+  // advances t and loops every step; diag holds the z pass's per-block
+  // partials (kDiagPart doubles per block, nbB blocks), then the record
+  // count, the CFL breach flag and the series of kDiagRec-double records
+  double t;
+  long long loops;
+  int nbB;
+  double* diag;
 };
+
+constexpr int kDiagPart = 4;   // max ux, max uz, sum(ux^2 + uz^2), unused
+constexpr int kDiagRec = 7;    // loop, t, dt of the step, KE, max ux, max uz, next dt
+constexpr int kDiagCap = 4096; // records kept per job (the oldest are overwritten)
+__host__ __device__ constexpr size_t diag_doubles(int nbB) {
+  return (size_t)kDiagPart * nbB + 4 + (size_t)kDiagRec * kDiagCap;
+}
 
 // bfloat16 pair <-> float2 (round to nearest even; the residuals are finite)
 __device__ __forceinline__ float2 bf2_load(uint32_t v) {
@@ -97,6 +112,7 @@ template <int V> struct BProg;
 template <int NF> struct BVortProg {
   static constexpr int NIN = NF + 2;  // fields, ux, uz
   static constexpr int NEQ = NF;
+  static constexpr int UX = NF, UZ = NF + 1;  // velocity channels
   template <typename T>
   __device__ __forceinline__ static void products(const T* v, T* px, T* pz) {
 #pragma unroll
@@ -112,6 +128,7 @@ template <> struct BProg<BV_VORT3> : BVortProg<3> {};
 // MHD channels: 0 w, 1 j, 2 ux, 3 uz, 4 bx, 5 bz
 template <> struct BProg<BV_MHD1> {
   static constexpr int NIN = 6;
+  static constexpr int UX = 2, UZ = 3;
   static constexpr int NEQ = 1;
   template <typename T>
   __device__ __forceinline__ static void products(const T* v, T* px, T* pz) {
@@ -121,6 +138,7 @@ template <> struct BProg<BV_MHD1> {
 };
 template <> struct BProg<BV_MHD2> {
   static constexpr int NIN = 6;
+  static constexpr int UX = 2, UZ = 3;
   static constexpr int NEQ = 1;
   template <typename T>
   __device__ __forceinline__ static void products(const T* v, T* px, T* pz) {
@@ -191,10 +209,16 @@ struct BatchLaunch {
 // lengths and of the ping-pong period), so a captured sequence of 4k steps
 // replays correctly.
 template <typename T>
-hipError_t fused_stage(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int wphys,
+hipError_t fused_stage(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int flags,
                        int step, const cx<T>* tw, hipStream_t s, hipEvent_t* ev = nullptr);
 template <typename T>
 hipError_t fused_set_dt(JobDev<T>* jobs, int job, double dt, hipStream_t s);
+// flags of a stage's z pass: physical outputs, diagnostics partials
+constexpr int kWritePhys = 1, kDiag = 2;
+// After a step whose z pass ran with kDiag: reduce each job's partials,
+// append a tracker record, and apply the CFL rule to the job's dt.
+template <typename T>
+hipError_t fused_diag(JobDev<T>* jobs, int njobs, double cutoff, hipStream_t s);
 template <typename T>
 hipError_t fused_prepare(const BatchLaunch& bl);
 

@@ -183,7 +183,12 @@ class Job:
             for i, p in enumerate(self.phys):
                 d.phys[i] = p.data_ptr()
         d.curr, d.parity = self.curr, self.parity
+        d.t, d.loops = self.t, self.steps
         return d
+
+
+# Columns of the tracker records written by the on-device CFL check.
+TRACKER_COLUMNS = ("loop", "t", "dt", "kinetic_energy", "max_ux", "max_uz", "next_dt")
 
 
 class Batch:
@@ -236,6 +241,45 @@ class Batch:
         _lib.call("mlv_batch_set_dt", self.handle, int(index), float(dt),
                   _lib.stream_handle(stream))
 
+    def run_device(self, nsteps, cadence=10, cutoff=0.5, graph=False, stream=None):
+        """Queue ``nsteps`` loops with the CFL ticker applied on the device
+        (no host synchronisation; see mlv_batch_run).  Host-side dt and t of
+        the jobs are stale until ``sync()``."""
+        if nsteps <= 0:
+            return
+        if graph and (self.jobs[0].steps - self._steps0[0]) % 4:
+            raise ValueError("graph replay needs the batch step count at a multiple of 4")
+        _lib.call("mlv_batch_run", self.handle, int(nsteps), int(cadence), float(cutoff),
+                  1 if graph else 0, _lib.stream_handle(stream))
+        for j in self.jobs:
+            j.steps += nsteps
+            j.curr = (j.curr + nsteps) % j.ab_order
+            j.parity = (j.parity + nsteps) & 1
+
+    def sync(self):
+        """Read every job's device-side dt and t; returns the indices of
+        jobs whose CFL check found a breach."""
+        breached = []
+        for i, j in enumerate(self.jobs):
+            dt, t = ctypes.c_double(), ctypes.c_double()
+            loops, br = ctypes.c_longlong(), ctypes.c_int()
+            _lib.call("mlv_batch_job_time", self.handle, i, ctypes.byref(dt), ctypes.byref(t),
+                      ctypes.byref(loops), ctypes.byref(br))
+            j.dt, j.t = dt.value, t.value
+            if br.value:
+                breached.append(i)
+        return breached
+
+    def tracker(self, index, max_records=4096):
+        """The job's tracker records (oldest first) as a dict of arrays
+        named by TRACKER_COLUMNS."""
+        buf = (ctypes.c_double * (7 * max_records))()
+        n = ctypes.c_int()
+        _lib.call("mlv_batch_tracker", self.handle, int(index), buf, int(max_records),
+                  ctypes.byref(n))
+        a = np.frombuffer(buf, dtype=np.float64)[: 7 * n.value].reshape(n.value, 7)
+        return {name: a[:, k].copy() for k, name in enumerate(TRACKER_COLUMNS)}
+
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:
             _lib.lib().mlv_batch_destroy(self.handle)
@@ -266,6 +310,35 @@ def cfl_dt(job, dt, cutoff=0.5):
     while dt > cutoff * c:
         dt = dt * 0.9
     return dt
+
+
+def run_device(batch, nsteps, cfl_cadence=10, cutoff=0.5, chunk=None):
+    """``run`` with the CFL ticker on the device: captured graphs of
+    ``chunk`` loops (a multiple of 4 and of the cadence) replayed back to
+    back, eager steps for what does not fit, one synchronisation at the
+    end.  Raises as the reference does if any job's check found a breach
+    (the device cannot stop mid-run, so the jobs' states after the breach
+    are not meaningful)."""
+    import math as _m
+
+    unit = 4 * cfl_cadence // _m.gcd(4, cfl_cadence) if cfl_cadence else 4
+    chunk = chunk or unit * max(1, 100 // unit)
+    if chunk % unit:
+        raise ValueError(f"chunk must be a multiple of {unit}")
+    done = 0
+    while done < nsteps:
+        aligned = (batch.jobs[0].steps - batch._steps0[0]) % unit == 0
+        if aligned and nsteps - done >= chunk:
+            batch.run_device(chunk, cfl_cadence, cutoff, graph=True)
+            done += chunk
+        else:
+            k = nsteps - done if aligned else min(nsteps - done,
+                                                   unit - (batch.jobs[0].steps - batch._steps0[0]) % unit)
+            batch.run_device(k, cfl_cadence, cutoff, graph=False)
+            done += k
+    if batch.sync():
+        raise Exception("CFL condition breached")
+    return done
 
 
 def run(batch, nsteps, cfl_cadence=10, cutoff=0.5, on_cfl=None, cfl=None):
