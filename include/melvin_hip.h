@@ -53,6 +53,11 @@ extern "C" {
 #define MLV_SPEC_SNABLA2 5
 #define MLV_SPEC_POISSON 6
 
+/* bases of the spectral axes (BasisFunctions.py:5-9) */
+#define MLV_BASIS_COMPLEX_EXP 0
+#define MLV_BASIS_COSINE 1
+#define MLV_BASIS_SINE 2
+
 /* physical finite differences (SpatialDifferentiator.py:76-185) */
 #define MLV_FD_DDX 0
 #define MLV_FD_DDZ 1
@@ -107,6 +112,22 @@ int mlv_to_spectral(mlv_plan* plan, const void* phys, void* spec, void* stream);
  * FD-z, MLV_SPEC_D2DZ2 / MLV_SPEC_SNABLA2 use the FD z-derivative
  * (SpatialDifferentiator.py:35-40). */
 int mlv_spec_op(mlv_plan* plan, int kind, const void* in, void* out, void* stream);
+/* The same three calls with a basis per axis (MLV_BASIS_COMPLEX_EXP, _COSINE
+ * or _SINE; fully spectral plans only): SpectralTransformer.to_spectral /
+ * to_physical with basis_functions (SpectralTransformer.py:90-199) and the
+ * spectral operators with the factors of BasisFunctions.py:26-59 (bx for the
+ * x factors, bz for the z factors; MLV_SPEC_POISSON divides by that basis's
+ * Laplacian symbol, LaplacianSolver.py:58-68).  Cosine / sine axes are dense
+ * per-axis transforms of the mirrored signal, built once per plan and basis.
+ * to_physical does not double the caller's k = 0 cosine terms in place as
+ * the reference does (SpectralTransformer.py:117-120); its result is that of
+ * the reference. */
+int mlv_to_spectral_basis(mlv_plan* plan, int bx, int bz, const void* phys, void* spec,
+                          void* stream);
+int mlv_to_physical_basis(mlv_plan* plan, int bx, int bz, const void* spec, void* phys,
+                          void* stream);
+int mlv_spec_op_basis(mlv_plan* plan, int kind, int bx, int bz, const void* in, void* out,
+                      void* stream);
 /* SpatialDifferentiator.pddx/pddz/pd2dz2 on a rows x cols array of real
  * (is_complex = 0) or complex values (SpatialDifferentiator.py:76-185). */
 int mlv_fd_op(mlv_plan* plan, int kind, int is_complex, int rows, int cols, const void* in,

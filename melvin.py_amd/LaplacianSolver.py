@@ -64,6 +64,7 @@ class LaplacianSolver:
         self._xp = hip
         self._array_factory = array_factory
         self._plan = plan_for(params)
+        self._bases = tuple(int(b) for b in basis_fns) if basis_fns is not None else (0, 0)
         if params.is_fully_spectral():
             if spatial_diff is not None and basis_fns is not None:
                 self.lap = spatial_diff.calc_lap(basis_fns)
@@ -82,6 +83,14 @@ class LaplacianSolver:
         return finish(dst, ret)
 
     def _solve_fully_spectral(self, rhs, out=None):
+        if any(b in (1, 2) for b in self._bases):
+            # rhs / lap of the cosine / sine factors (LaplacianSolver.py:58-68)
+            pl = self._plan
+            src = as_input(rhs, pl.cplx, (pl.rows, pl.cols))
+            dst, ret = output(out, pl.cplx, (pl.rows, pl.cols))
+            _lib.call("mlv_spec_op_basis", pl.handle, _lib.SPEC_POISSON, self._bases[0],
+                      self._bases[1], ptr(src), ptr(dst), _lib.stream_handle())
+            return finish(dst, ret)
         return self._solve(rhs, out)
 
     def _solve_fdm(self, rhs, out=None):

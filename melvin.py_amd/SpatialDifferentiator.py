@@ -39,45 +39,54 @@ class SpatialDifferentiator:
             self.sd2dz2 = lambda var, bs=None: self.pd2dz2(var)
 
     # -- spectral ---------------------------------------------------------
-    def _spec(self, kind, var, basis_fn):
+    def _spec(self, kind, var, bx, bz=BasisFunctions.COMPLEX_EXP):
+        """One spectral multiply; bx / bz are the bases whose factors the
+        operator uses (BasisFunctions.py:26-59)."""
         pl = self._plan
-        if basis_fn in (BasisFunctions.COSINE, BasisFunctions.SINE):
-            raise NotImplementedError("cosine/sine bases are not implemented")
         src = as_input(var, pl.cplx, (pl.rows, pl.cols))
         out = torch.empty_like(src)
-        if basis_fn is BasisFunctions.FDM:  # zero derivative factor
-            out.zero_()
+        uses_x = kind in (_lib.SPEC_DDX, _lib.SPEC_D2DX2)
+        uses_z = kind in (_lib.SPEC_DDZ, _lib.SPEC_D2DZ2)
+        if (uses_x and bx is BasisFunctions.FDM) or (uses_z and bz is BasisFunctions.FDM):
+            out.zero_()  # the FDM basis has a zero derivative factor
             return out
-        _lib.call("mlv_spec_op", pl.handle, kind, ptr(src), ptr(out),
-                  _lib.stream_handle())
+        if int(bx) in (1, 2) or int(bz) in (1, 2):
+            _lib.call("mlv_spec_op_basis", pl.handle, kind, int(bx), int(bz), ptr(src),
+                      ptr(out), _lib.stream_handle())
+        else:
+            _lib.call("mlv_spec_op", pl.handle, kind, ptr(src), ptr(out),
+                      _lib.stream_handle())
         return out
 
     def sddx(self, var, basis_fn=BasisFunctions.COMPLEX_EXP):
         return self._spec(_lib.SPEC_DDX, var, basis_fn)
 
     def sddz(self, var, basis_fn=BasisFunctions.COMPLEX_EXP):
-        return self._spec(_lib.SPEC_DDZ, var, basis_fn)
+        return self._spec(_lib.SPEC_DDZ, var, BasisFunctions.COMPLEX_EXP, basis_fn)
 
     def sd2dx2(self, var, basis_fn=BasisFunctions.COMPLEX_EXP):
         return self._spec(_lib.SPEC_D2DX2, var, basis_fn)
 
     def sd2dz2(self, var, basis_fn=BasisFunctions.COMPLEX_EXP):
-        return self._spec(_lib.SPEC_D2DZ2, var, basis_fn)
+        return self._spec(_lib.SPEC_D2DZ2, var, BasisFunctions.COMPLEX_EXP, basis_fn)
 
     def snabla2(self, var, basis_fns=(BasisFunctions.COMPLEX_EXP,
                                       BasisFunctions.COMPLEX_EXP)):
         """sd2dx2 + sd2dz2 in one pass (Variable.py:111-113)."""
-        return self._spec(_lib.SPEC_SNABLA2, var, basis_fns[0])
+        return self._spec(_lib.SPEC_SNABLA2, var, basis_fns[0], basis_fns[1])
 
     def calc_lap(self, basis_fns=(BasisFunctions.COMPLEX_EXP,
                                   BasisFunctions.COMPLEX_EXP)):
         """float64 symbol -(kx^2) - (kz^2) (SpatialDifferentiator.py:70-74)."""
         pl = self._plan
-        if any(b in (BasisFunctions.COSINE, BasisFunctions.SINE) for b in basis_fns):
-            raise NotImplementedError("cosine/sine bases are not implemented")
         out = torch.empty((pl.rows, pl.cols), dtype=torch.float64, device=hip.device())
-        _lib.call("mlv_spec_op", pl.handle, _lib.SPEC_LAP, ctypes_null(), ptr(out),
-                  _lib.stream_handle())
+        b0, b1 = (int(b) for b in basis_fns)
+        if b0 in (1, 2) or b1 in (1, 2):
+            _lib.call("mlv_spec_op_basis", pl.handle, _lib.SPEC_LAP, b0, b1, ctypes_null(),
+                      ptr(out), _lib.stream_handle())
+        else:
+            _lib.call("mlv_spec_op", pl.handle, _lib.SPEC_LAP, ctypes_null(), ptr(out),
+                      _lib.stream_handle())
         return out
 
     # -- physical finite differences -------------------------------------

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -132,6 +133,7 @@ struct mlv_plan {
   double* rden;
   double lower;
   double* red_part;
+  std::map<int, void*> basis_mats;  // dense per-axis transforms, built on first use
 };
 
 extern "C" {
@@ -158,8 +160,7 @@ int mlv_plan_create(mlv_plan** out, int nx, int nz, double lx, double lz, int di
   } else if (nz < 5) {
     return fail(MLV_ERR_UNSUPPORTED, "FD-z needs nz >= 5");
   }
-  mlv_plan* p = new mlv_plan();
-  std::memset(p, 0, sizeof(*p));
+  mlv_plan* p = new mlv_plan();  // value-initialised: scalars and pointers zero
   HIP_TRY(hipGetDevice(&p->device), "hipGetDevice");
   p->nx = nx;
   p->nz = nz;
@@ -243,6 +244,7 @@ int mlv_plan_destroy(mlv_plan* p) {
   if (p->cp) (void)hipFree(p->cp);
   if (p->rden) (void)hipFree(p->rden);
   if (p->red_part) (void)hipFree(p->red_part);
+  for (auto& kv : p->basis_mats) (void)hipFree(kv.second);
   delete p;
   return MLV_OK;
 }
@@ -1079,6 +1081,190 @@ int mlv_batch_set_dt(mlv_batch* bp, int job, double dt, void* stream) {
 long long mlv_batch_steps(const mlv_batch* bp) {
   if (!bp) return -1;
   return reinterpret_cast<const BatchHost*>(bp)->steps;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Cosine and sine bases (SpectralTransformer.py:90-199, BasisFunctions.py:
+// 26-59).  The reference mirrors the signal about its end point to length
+// 2(n-1) (sine: with a sign flip) and takes an FFT of that length, dividing
+// by n-1 (sine: -i(n-1)) and halving the k = 0 cosine term; the inverse
+// doubles that term, multiplies by the same factor, inverse-transforms and
+// crops to n points.  2(n-1) is rarely a power of two, so each axis is a
+// dense transform whose matrix folds the mirror images, the factors and the
+// crop together; the matrices are built once per plan in long double.
+// ---------------------------------------------------------------------------
+namespace {
+
+using cld = std::complex<long double>;
+constexpr int kExp = MLV_BASIS_COMPLEX_EXP, kCos = MLV_BASIS_COSINE, kSin = MLV_BASIS_SINE;
+const long double kPiL = 3.141592653589793238462643383279502884L;
+
+cld cis(long double a) { return cld(cosl(a), sinl(a)); }
+
+// length of the periodic sequence the reference transforms along one axis
+int axis_period(int b, int n) { return b == kExp ? n : 2 * (n - 1); }
+// the reference's axis factor: n, n-1 or -i(n-1)
+cld axis_factor(int b, int n) {
+  if (b == kExp) return cld((long double)n, 0);
+  if (b == kCos) return cld((long double)(n - 1), 0);
+  return cld(0, -(long double)(n - 1));
+}
+
+// Forward: spectral value of mode k (signed) from sample j of n.
+cld fwd_entry(int b, int n, int k, int j) {
+  const int L = axis_period(b, n);
+  cld v = 0;
+  if (b == kExp) {
+    v = cis(-2 * kPiL * (long double)j * k / L);
+  } else {
+    if (j <= n - 2) v += cis(-2 * kPiL * (long double)j * k / L);
+    if (j >= 1) v += (b == kCos ? 1.0L : -1.0L) * cis(-2 * kPiL * (long double)(L - j) * k / L);
+  }
+  v /= axis_factor(b, n);
+  if (b == kCos && k == 0) v /= 2.0L;
+  return v;
+}
+
+// Inverse along the full complex axis (x): sample j from mode k (signed).
+cld inv_entry(int b, int n, int k, int j) {
+  const int L = axis_period(b, n);
+  cld v = cis(2 * kPiL * (long double)j * k / L) / (long double)L * axis_factor(b, n);
+  if (b == kCos && k == 0) v *= 2.0L;
+  return v;
+}
+
+// Inverse along the half-spectrum axis (z, real output): the weight of
+// mode m in sample j, so that sample = Re(sum_m W U_m) as irfft gives it.
+cld inv_half_entry(int b, int n, int m, int j) {
+  const int L = axis_period(b, n);
+  cld v = cis(2 * kPiL * (long double)j * m / L) / (long double)L * axis_factor(b, n);
+  if (m > 0) v *= 2.0L;
+  if (b == kCos && m == 0) v *= 2.0L;
+  return v;
+}
+
+int mode_of_row(const mlv_plan* p, int r) { return r <= p->nn ? r : r - p->rows; }
+
+// Matrix key: direction (0 forward, 1 inverse), axis (0 x, 1 z), basis.
+int get_basis_mat(mlv_plan* p, int dir, int axis, int b, const void** out) {
+  const int key = dir * 16 + axis * 4 + b;
+  auto it = p->basis_mats.find(key);
+  if (it != p->basis_mats.end()) {
+    *out = it->second;
+    return MLV_OK;
+  }
+  int K, J;
+  if (dir == 0 && axis == 1) { K = p->nm; J = p->nz; }
+  else if (dir == 0) { K = p->rows; J = p->nx; }
+  else if (axis == 0) { K = p->nx; J = p->rows; }
+  else { K = p->nz; J = p->nm; }
+  std::vector<cld> m((size_t)K * J);
+  for (int k = 0; k < K; ++k)
+    for (int j = 0; j < J; ++j) {
+      cld v;
+      if (dir == 0 && axis == 1) v = fwd_entry(b, p->nz, k, j);
+      else if (dir == 0) v = fwd_entry(b, p->nx, mode_of_row(p, k), j);
+      else if (axis == 0) v = inv_entry(b, p->nx, mode_of_row(p, j), k);
+      else v = inv_half_entry(b, p->nz, j, k);
+      m[(size_t)k * J + j] = v;
+    }
+  const size_t nel = (size_t)K * J;
+  void* d = nullptr;
+  if (p->dtype == MLV_F64) {
+    std::vector<double> h(2 * nel);
+    for (size_t i = 0; i < nel; ++i) { h[2 * i] = (double)m[i].real(); h[2 * i + 1] = (double)m[i].imag(); }
+    HIP_TRY(hipMalloc(&d, h.size() * sizeof(double)), "hipMalloc basis matrix");
+    HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice), "upload");
+  } else {
+    std::vector<float> h(2 * nel);
+    for (size_t i = 0; i < nel; ++i) { h[2 * i] = (float)m[i].real(); h[2 * i + 1] = (float)m[i].imag(); }
+    HIP_TRY(hipMalloc(&d, h.size() * sizeof(float)), "hipMalloc basis matrix");
+    HIP_TRY(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice), "upload");
+  }
+  p->basis_mats[key] = d;
+  *out = d;
+  return MLV_OK;
+}
+
+int check_bases(const mlv_plan* p, int bx, int bz) {
+  if (!p->fully) return fail(MLV_ERR_UNSUPPORTED, "cosine/sine bases need a fully spectral plan");
+  if (bx < kExp || bx > kSin || bz < kExp || bz > kSin) return fail(MLV_ERR_ARG, "unknown basis");
+  return MLV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mlv_to_spectral_basis(mlv_plan* p, int bx, int bz, const void* phys, void* spec, void* stream) {
+  CHECK_PLAN(p);
+  CHECK_PTR(phys);
+  CHECK_PTR(spec);
+  int rc = check_bases(p, bx, bz);
+  if (rc) return rc;
+  if (bx == kExp && bz == kExp) return mlv_to_spectral(p, phys, spec, stream);
+  const void *mz, *mx;
+  if ((rc = get_basis_mat(p, 0, 1, bz, &mz)) || (rc = get_basis_mat(p, 0, 0, bx, &mx))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  // z: (nx, nz) real -> (nx, nm) complex in the plan workspace
+  hipError_t e = DISPATCH(p,
+      launch_dense<float>(mz, p->nm, p->nz, p->nx, phys, 0, p->nz, 1, p->work, 0, p->nm, 1, s),
+      launch_dense<double>(mz, p->nm, p->nz, p->nx, phys, 0, p->nz, 1, p->work, 0, p->nm, 1, s));
+  // x: per kz column, (nx) -> (rows) complex
+  if (e == hipSuccess)
+    e = DISPATCH(p,
+        launch_dense<float>(mx, p->rows, p->nx, p->nm, p->work, 1, 1, p->nm, spec, 0, 1, p->nm, s),
+        launch_dense<double>(mx, p->rows, p->nx, p->nm, p->work, 1, 1, p->nm, spec, 0, 1, p->nm, s));
+  if (e != hipSuccess) return hip_fail(e, "to_spectral (basis)");
+  return MLV_OK;
+}
+
+int mlv_to_physical_basis(mlv_plan* p, int bx, int bz, const void* spec, void* phys, void* stream) {
+  CHECK_PLAN(p);
+  CHECK_PTR(phys);
+  CHECK_PTR(spec);
+  int rc = check_bases(p, bx, bz);
+  if (rc) return rc;
+  if (bx == kExp && bz == kExp) return mlv_to_physical(p, spec, phys, stream);
+  const void *mx, *mz;
+  if ((rc = get_basis_mat(p, 1, 0, bx, &mx)) || (rc = get_basis_mat(p, 1, 1, bz, &mz))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  // x: per kz column, (rows) -> (nx) complex
+  hipError_t e = DISPATCH(p,
+      launch_dense<float>(mx, p->nx, p->rows, p->nm, spec, 1, 1, p->nm, p->work, 0, 1, p->nm, s),
+      launch_dense<double>(mx, p->nx, p->rows, p->nm, spec, 1, 1, p->nm, p->work, 0, 1, p->nm, s));
+  // z: per x row, (nm) complex -> (nz) real
+  if (e == hipSuccess)
+    e = DISPATCH(p,
+        launch_dense<float>(mz, p->nz, p->nm, p->nx, p->work, 1, p->nm, 1, phys, 1, p->nz, 1, s),
+        launch_dense<double>(mz, p->nz, p->nm, p->nx, p->work, 1, p->nm, 1, phys, 1, p->nz, 1, s));
+  if (e != hipSuccess) return hip_fail(e, "to_physical (basis)");
+  return MLV_OK;
+}
+
+int mlv_spec_op_basis(mlv_plan* p, int kind, int bx, int bz, const void* in, void* out,
+                      void* stream) {
+  CHECK_PLAN(p);
+  CHECK_PTR(out);
+  if (kind != MLV_SPEC_LAP) CHECK_PTR(in);
+  int rc = check_bases(p, bx, bz);
+  if (rc) return rc;
+  if (kind < MLV_SPEC_DDX || kind > MLV_SPEC_POISSON) return fail(MLV_ERR_ARG, "unknown spectral op");
+  // BasisFunctions.py:26-59: first-derivative wavelength i 2 pi (exp), pi
+  // (sine), -pi (cosine); second derivative -|wavelength|^2; over L, L^2
+  auto w1 = [](int b) { return b == kExp ? 2.0 * M_PI : (b == kSin ? M_PI : -M_PI); };
+  auto w2 = [](int b) { const double a = b == kExp ? 2.0 * M_PI : M_PI; return -(a * a); };
+  const double fx = w1(bx) / p->lx, fz = w1(bz) / p->lz;
+  const double fx2 = w2(bx) / (p->lx * p->lx), fz2 = w2(bz) / (p->lz * p->lz);
+  const int real = (bx != kExp ? 1 : 0) | (bz != kExp ? 2 : 0);
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = DISPATCH(p,
+      launch_spec<float>(in, out, p->rows, p->cols, p->nn, 1, kind, fx, fz, fx2, fz2, s, real),
+      launch_spec<double>(in, out, p->rows, p->cols, p->nn, 1, kind, fx, fz, fx2, fz2, s, real));
+  if (e != hipSuccess) return hip_fail(e, "spectral op (basis)");
+  return MLV_OK;
 }
 
 }  // extern "C"

@@ -22,7 +22,8 @@ __device__ __forceinline__ int mode_x(int r, int rows, int nn, int fully) {
 template <typename T>
 __global__ void __launch_bounds__(kEW) k_spec(const cx<T>* __restrict__ in, void* __restrict__ out,
                                               int rows, int cols, int nn, int fully, int kind,
-                                              double fx, double fz, double fx2, double fz2) {
+                                              double fx, double fz, double fx2, double fz2,
+                                              int real) {
   const size_t total = (size_t)rows * cols;
   for (size_t e = blockIdx.x * (size_t)kEW + threadIdx.x; e < total;
        e += (size_t)gridDim.x * kEW) {
@@ -39,12 +40,12 @@ __global__ void __launch_bounds__(kEW) k_spec(const cx<T>* __restrict__ in, void
     switch (kind) {
       case MLV_SPEC_DDX: {
         const T y = (T)(fx * n);
-        o = mk<T>(-(y * v.y), y * v.x);
+        o = (real & 1) ? mk<T>(y * v.x, y * v.y) : mk<T>(-(y * v.y), y * v.x);
         break;
       }
       case MLV_SPEC_DDZ: {
         const T y = (T)(fz * m);
-        o = mk<T>(-(y * v.y), y * v.x);
+        o = (real & 2) ? mk<T>(y * v.x, y * v.y) : mk<T>(-(y * v.y), y * v.x);
         break;
       }
       case MLV_SPEC_D2DX2: {
@@ -404,9 +405,59 @@ static inline unsigned ew_grid(size_t n) {
 
 template <typename T>
 hipError_t launch_spec(const void* in, void* out, int rows, int cols, int nn, int fully, int kind,
-                       double fx, double fz, double fx2, double fz2, hipStream_t s) {
+                       double fx, double fz, double fx2, double fz2, hipStream_t s, int real) {
   hipLaunchKernelGGL(k_spec<T>, dim3(ew_grid((size_t)rows * cols)), dim3(kEW), 0, s,
-                     (const cx<T>*)in, out, rows, cols, nn, fully, kind, fx, fz, fx2, fz2);
+                     (const cx<T>*)in, out, rows, cols, nn, fully, kind, fx, fz, fx2, fz2, real);
+  return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------
+// Dense per-axis transforms (the cosine / sine bases): one thread per output
+// element, the sum over j in the precision of T (FP64 for double plans).
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(kEW) k_dense(const cx<T>* __restrict__ M, int K, int J, int batch,
+                                               const void* __restrict__ in, int in_cplx,
+                                               long long in_sb, long long in_sj,
+                                               void* __restrict__ out, int out_real,
+                                               long long out_sb, long long out_sk) {
+  const size_t total = (size_t)K * batch;
+  for (size_t e = blockIdx.x * (size_t)kEW + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * kEW) {
+    const int k = (int)(e % K), b = (int)(e / K);
+    const cx<T>* __restrict__ row = M + (size_t)k * J;
+    T re = T(0), im = T(0);
+    if (in_cplx) {
+      const cx<T>* __restrict__ x = reinterpret_cast<const cx<T>*>(in) + (size_t)b * in_sb;
+      for (int j = 0; j < J; ++j) {
+        const cx<T> m = row[j], v = x[(size_t)j * in_sj];
+        re += m.x * v.x - m.y * v.y;
+        im += m.x * v.y + m.y * v.x;
+      }
+    } else {
+      const T* __restrict__ x = reinterpret_cast<const T*>(in) + (size_t)b * in_sb;
+      for (int j = 0; j < J; ++j) {
+        const cx<T> m = row[j];
+        const T v = x[(size_t)j * in_sj];
+        re += m.x * v;
+        im += m.y * v;
+      }
+    }
+    const size_t o = (size_t)b * out_sb + (size_t)k * out_sk;
+    if (out_real)
+      reinterpret_cast<T*>(out)[o] = re;
+    else
+      reinterpret_cast<cx<T>*>(out)[o] = mk<T>(re, im);
+  }
+}
+
+template <typename T>
+hipError_t launch_dense(const void* M, int K, int J, int batch, const void* in, int in_cplx,
+                        long long in_sb, long long in_sj, void* out, int out_real,
+                        long long out_sb, long long out_sk, hipStream_t s) {
+  hipLaunchKernelGGL(k_dense<T>, dim3(ew_grid((size_t)K * batch)), dim3(kEW), 0, s,
+                     (const cx<T>*)M, K, J, batch, in, in_cplx, in_sb, in_sj, out, out_real,
+                     out_sb, out_sk);
   return hipGetLastError();
 }
 
@@ -468,7 +519,9 @@ hipError_t launch_velocity(const void* w, void* psi, void* ux, void* uz, int row
 
 #define MLV_INST(T)                                                                              \
   template hipError_t launch_spec<T>(const void*, void*, int, int, int, int, int, double, double, \
-                                     double, double, hipStream_t);                               \
+                                     double, double, hipStream_t, int);                          \
+  template hipError_t launch_dense<T>(const void*, int, int, int, const void*, int, long long,    \
+                                      long long, void*, int, long long, long long, hipStream_t); \
   template hipError_t launch_update<T>(void*, void*, const void*, size_t, int, int, int, double,  \
                                        double, double, hipStream_t);                             \
   template hipError_t launch_fd<T>(const void*, void*, int, int, int, int, int, int, double,      \

@@ -9,9 +9,19 @@ namespace mlv {
 // internal spectral kind: out += d2/dx2(in) (FD-z snabla2)
 constexpr int kSpecAddD2dx2 = 100;
 
+// real: bit 0 / bit 1 make the x / z first-derivative factor real (sine and
+// cosine bases) instead of i times the factor (complex exponentials)
 template <typename T>
 hipError_t launch_spec(const void* in, void* out, int rows, int cols, int nn, int fully, int kind,
-                       double fx, double fz, double fx2, double fz2, hipStream_t s);
+                       double fx, double fz, double fx2, double fz2, hipStream_t s,
+                       int real = 0);
+// Batched dense complex matrix-vector products, out[b, k] = sum_j M[k, j] in[b, j]
+// (M row-major K x J); the input may be real, the output reduced to its real
+// part.  Strides in elements.
+template <typename T>
+hipError_t launch_dense(const void* M, int K, int J, int batch, const void* in, int in_cplx,
+                        long long in_sb, long long in_sj, void* out, int out_real,
+                        long long out_sb, long long out_sk, hipStream_t s);
 template <typename T>
 hipError_t launch_update(void* var, void* hist, const void* lin, size_t n, int q, int curr,
                          int scheme, double c1, double c2, double h, hipStream_t s);

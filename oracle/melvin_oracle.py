@@ -163,6 +163,115 @@ def to_spectral(p: Params, phys: np.ndarray) -> np.ndarray:
 
 
 # --------------------------------------------------------------------------
+# Cosine and sine bases (SpectralTransformer.py:90-199, BasisFunctions.py)
+# --------------------------------------------------------------------------
+def _basis_period(b, n):
+    """Length of the periodic signal a basis transforms along an axis of n
+    samples: n for COMPLEX_EXP, the 2(n-1) of the mirror image otherwise."""
+    return n if b == COMPLEX_EXP else 2 * (n - 1)
+
+
+def _basis_norm(b, n):
+    """Normalisation of a basis's coefficients (SpectralTransformer.py:104-115,
+    167-180): n, n-1 (cosine) or -i(n-1) (sine)."""
+    return {COMPLEX_EXP: n, COSINE: n - 1, SINE: -1j * (n - 1)}[b]
+
+
+def _mirror(a, b, axis):
+    """Extend ``a`` along ``axis`` to its basis period: samples 0..n-2, then
+    n-1..1 reflected (negated for the sine basis)."""
+    if b == COMPLEX_EXP:
+        return a
+    n = a.shape[axis]
+    head = np.take(a, np.arange(0, n - 1), axis=axis)
+    tail = np.take(a, np.arange(n - 1, 0, -1), axis=axis)
+    return np.concatenate((head, -tail if b == SINE else tail), axis=axis)
+
+
+def to_spectral_basis(p: Params, phys: np.ndarray, bases) -> np.ndarray:
+    """Fully spectral forward transform with a basis per axis
+    (SpectralTransformer.py:152-199): the signal mirrored to its basis
+    period on each cosine / sine axis, ``rfft2``, divided by the two
+    normalisations, the k = 0 cosine coefficients halved, then truncated as
+    ``to_spectral``."""
+    bx, bz = bases
+    ext = _mirror(_mirror(np.asarray(phys, dtype=np.float64), bx, 0), bz, 1)
+    f = np.fft.rfft2(ext) / (_basis_norm(bx, p.nx) * _basis_norm(bz, p.nz))
+    if bx == COSINE:
+        f[0, :] *= 0.5
+    if bz == COSINE:
+        f[:, 0] *= 0.5
+    out = np.zeros(p.spectral_shape, dtype=np.complex128)
+    out[: p.nn + 1] = f[: p.nn + 1, : p.nm]
+    out[-p.nn:] = f[-p.nn:, : p.nm]
+    return out
+
+
+def to_physical_basis(p: Params, spec: np.ndarray, bases) -> np.ndarray:
+    """Inverse of ``to_spectral_basis`` (SpectralTransformer.py:90-150):
+    k = 0 cosine coefficients doubled, the compact spectrum placed in the
+    half-spectrum of the basis periods, scaled by both normalisations,
+    ``irfft2`` and cropped to the first nx x nz samples.  The input is not
+    modified (the reference doubles its caller's k = 0 cosine rows and
+    columns in place, SpectralTransformer.py:117-120)."""
+    bx, bz = bases
+    lx_, lz_ = _basis_period(bx, p.nx), _basis_period(bz, p.nz)
+    s = np.array(spec, dtype=np.complex128)
+    if bx == COSINE:
+        s[0, :] *= 2
+    if bz == COSINE:
+        s[:, 0] *= 2
+    up = np.zeros((lx_, lz_ // 2 + 1), dtype=np.complex128)
+    up[: p.nn + 1, : p.nm] = s[: p.nn + 1]
+    up[-p.nn:, : p.nm] = s[-p.nn:]
+    up *= _basis_norm(bx, p.nx) * _basis_norm(bz, p.nz)
+    return np.fft.irfft2(up, s=(lx_, lz_))[: p.nx, : p.nz]
+
+
+def basis_diff_factor(b, length):
+    """First-derivative factor of a basis over a domain length
+    (BasisFunctions.py:26-37, 49-52): 2 pi i, pi (sine), -pi (cosine), 0
+    (FDM), over the length."""
+    w = {COMPLEX_EXP: 2j * np.pi, SINE: np.pi, COSINE: -np.pi}.get(b, 0.0)
+    return w / length
+
+
+def basis_diff2_factor(b, length):
+    """Second-derivative factor -|w|^2 / length^2 (BasisFunctions.py:40-43,
+    55-59)."""
+    return -np.abs(basis_diff_factor(b, 1.0)) ** 2 / length ** 2
+
+
+def spectral_op_basis(p: Params, kind: str, f, bases):
+    """The spectral operators with bases (SpatialDifferentiator.py:50-74,
+    Variable.py:111-113, LaplacianSolver.py:58-68): ``kind`` is one of
+    ddx, ddz, d2dx2, d2dz2, snabla2, lap (``f`` ignored) and poisson (f
+    divided by the Laplacian symbol, with 1 at mode (0, 0))."""
+    bx, bz = bases
+    n, m = mode_numbers(p)
+    d2x = basis_diff2_factor(bx, p.lx) * n ** 2
+    d2z = basis_diff2_factor(bz, p.lz) * m ** 2
+    if kind == "ddx":
+        return basis_diff_factor(bx, p.lx) * n * f
+    if kind == "ddz":
+        return basis_diff_factor(bz, p.lz) * m * f
+    if kind == "d2dx2":
+        return d2x * f
+    if kind == "d2dz2":
+        return d2z * f
+    if kind == "snabla2":
+        return d2x * f + d2z * f
+    lap = d2x + d2z
+    if kind == "lap":
+        return lap
+    if kind == "poisson":
+        lap = lap.copy()
+        lap[0, 0] = 1.0
+        return f / lap
+    raise ValueError(kind)
+
+
+# --------------------------------------------------------------------------
 # Spectral derivatives (SpatialDifferentiator.py:50-74)
 # --------------------------------------------------------------------------
 def sddx(p, f):

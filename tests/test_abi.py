@@ -67,6 +67,9 @@ def test_argument_errors_are_reported():
     with pytest.raises(NotImplementedError):
         L.call("mlv_plan_create", ctypes.byref(h), 32, 33, 1.0, 1.0, 0, 1, 2)
     assert lib.mlv_to_physical(None, None, None, None) == L.MLV_ERR_ARG
+    assert lib.mlv_to_spectral_basis(None, 1, 1, None, None, None) == L.MLV_ERR_ARG
+    assert lib.mlv_to_physical_basis(None, 1, 1, None, None, None) == L.MLV_ERR_ARG
+    assert lib.mlv_spec_op_basis(None, 0, 1, 1, None, None, None) == L.MLV_ERR_ARG
     assert lib.mlv_batch_step(None, 1, 0, None) == L.MLV_ERR_ARG
     assert lib.mlv_batch_steps(None) == -1
     n = ctypes.c_size_t()

@@ -321,3 +321,90 @@ def test_fdz_pc_and_theta_schemes_converge_at_second_order():
             e_si = errs[0]
         else:
             assert errs[0] < e_si / 5, (errs[0], e_si)
+
+
+# ---------------------------------------------------------------------------
+# Cosine and sine bases
+# ---------------------------------------------------------------------------
+import _basis_kats  # noqa: E402
+
+
+@pytest.mark.parametrize("name", sorted(_basis_kats.KATS))
+def test_kat_transform_bases(p256, name):
+    """SpectralTransformer_test.py:64-310: the seven cosine / sine cases,
+    forward to the listed coefficients and back to the field."""
+    bases, phys, want = _basis_kats.case(name)
+    spec = O.to_spectral_basis(p256, phys, bases)
+    np.testing.assert_array_almost_equal(spec, want)
+    np.testing.assert_array_almost_equal(O.to_physical_basis(p256, spec, bases), phys)
+
+
+def test_cosine_sine_forward_match_dct_dst_type1():
+    """A cosine axis is scipy's DCT-I / (n-1) (k = 0 halved), a sine axis
+    (field zero at both ends) scipy's DST-I of the interior / (n-1); the
+    other axis periodic and constant."""
+    import scipy.fft as sf
+
+    p = O.make_params(dict(nx=64, nz=16, lx=1.0, lz=1.0))
+    rng = np.random.default_rng(4)
+    f = rng.standard_normal(p.nx)
+    phys = np.repeat(f[:, None], p.nz, axis=1)
+    got = O.to_spectral_basis(p, phys, (O.COSINE, O.COMPLEX_EXP))[:, 0]
+    want = sf.dct(f, type=1) / (p.nx - 1)
+    want[0] *= 0.5
+    np.testing.assert_allclose(got[: p.nn + 1].real, want[: p.nn + 1], atol=1e-13)
+    np.testing.assert_allclose(got[: p.nn + 1].imag, 0, atol=1e-13)
+    g = f.copy()
+    g[0] = g[-1] = 0.0
+    phys = np.repeat(g[:, None], p.nz, axis=1)
+    got = O.to_spectral_basis(p, phys, (O.SINE, O.COMPLEX_EXP))[:, 0]
+    want = sf.dst(g[1:-1], type=1) / (p.nx - 1)
+    np.testing.assert_allclose(got[1: p.nn + 1].real, want[: p.nn], atol=1e-13)
+    np.testing.assert_allclose(got[1: p.nn + 1].imag, 0, atol=1e-13)
+    assert abs(got[0]) < 1e-13
+
+
+def test_basis_derivatives_closed_forms(p256):
+    """d/dx of a cosine series is the sine series with the factor -pi n
+    (and d/dz of a sine series the cosine series with pi m); second
+    derivatives keep the basis with -(pi n)^2: each checked in physical
+    space against the analytic derivative."""
+    x = np.linspace(0, 1.0, p256.nx)
+    z = np.linspace(0, 1.0, p256.nz)
+    X, Z = np.meshgrid(x, z, indexing="ij")
+    f = np.cos(np.pi * X) + 2.0 * np.cos(3 * np.pi * X)
+    s = O.to_spectral_basis(p256, f, (O.COSINE, O.COSINE))
+    d = O.spectral_op_basis(p256, "ddx", s, (O.COSINE, O.COSINE))
+    want = -np.pi * np.sin(np.pi * X) - 6.0 * np.pi * np.sin(3 * np.pi * X)
+    np.testing.assert_allclose(O.to_physical_basis(p256, d, (O.SINE, O.COSINE)), want,
+                               atol=1e-10)
+    g = np.sin(np.pi * Z) + 0.5 * np.sin(2 * np.pi * Z)
+    b = (O.COMPLEX_EXP, O.SINE)
+    s = O.to_spectral_basis(p256, g, b)
+    d = O.spectral_op_basis(p256, "ddz", s, b)
+    want = np.pi * np.cos(np.pi * Z) + np.pi * np.cos(2 * np.pi * Z)
+    np.testing.assert_allclose(O.to_physical_basis(p256, d, (O.COMPLEX_EXP, O.COSINE)), want,
+                               atol=1e-10)
+    d2 = O.spectral_op_basis(p256, "d2dz2", s, b)
+    want = -np.pi ** 2 * np.sin(np.pi * Z) - 2.0 * np.pi ** 2 * np.sin(2 * np.pi * Z)
+    np.testing.assert_allclose(O.to_physical_basis(p256, d2, b), want, atol=1e-9)
+
+
+def test_basis_poisson_inverts_the_laplacian(p256):
+    """sin(pi x) sin(2 pi z) solves lap(psi) = -(pi^2 + 4 pi^2) psi, and
+    poisson(lap * f) = f off mode (0, 0), for every basis pair."""
+    x = np.linspace(0, 1.0, p256.nx)
+    X, Z = np.meshgrid(x, x, indexing="ij")
+    f = np.sin(np.pi * X) * np.sin(2 * np.pi * Z)
+    b = (O.SINE, O.SINE)
+    rhs = O.to_spectral_basis(p256, -5 * np.pi ** 2 * f, b)
+    psi = O.spectral_op_basis(p256, "poisson", rhs, b)
+    np.testing.assert_allclose(O.to_physical_basis(p256, psi, b), f, atol=1e-12)
+    rng = np.random.default_rng(1)
+    s = rng.standard_normal(p256.spectral_shape) + 1j * rng.standard_normal(p256.spectral_shape)
+    s[0, 0] = 0
+    for bx in (O.COMPLEX_EXP, O.COSINE, O.SINE):
+        for bz in (O.COMPLEX_EXP, O.COSINE, O.SINE):
+            lap = O.spectral_op_basis(p256, "lap", None, (bx, bz))
+            back = O.spectral_op_basis(p256, "poisson", lap * s, (bx, bz))
+            np.testing.assert_allclose(back, s, rtol=1e-13, atol=1e-13)
