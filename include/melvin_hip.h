@@ -229,8 +229,11 @@ int mlv_batch_run(mlv_batch* batch, int nsteps, int cadence, double cutoff, int 
  * (synchronises the device). */
 int mlv_batch_job_time(mlv_batch* batch, int job, double* dt, double* t, long long* loops,
                        int* breach);
-/* Up to max_records tracker records of one job, oldest first, 7 doubles
- * each (see mlv_batch_run); the last 4096 are kept (synchronises). */
+/* Up to max_records tracker records of one job, oldest first, 9 doubles
+ * each: loop, t, dt of the step, kinetic energy, max ux, max uz, next dt,
+ * enstrophy 0.5 mean(w^2), and mean(f uz) of the job's first advected scalar
+ * f (NaN without one; double_diffusive_convection.py:20-23's Nusselt number
+ * is 1 minus it).  The last 4096 records are kept (synchronises). */
 int mlv_batch_tracker(mlv_batch* batch, int job, double* out, int max_records, int* n_records);
 /* Steps completed since creation (host count). */
 long long mlv_batch_steps(const mlv_batch* batch);
@@ -284,6 +287,8 @@ typedef struct mlv_fdz_job_desc {
                           state a step starts from, or NULL */
   int curr;            /* ring slot the next step writes */
   int parity;          /* state buffer holding the current state */
+  double t;            /* simulation time (advanced on the device) */
+  long long loops;     /* loop count (Simulation._loop_counter) */
 } mlv_fdz_job_desc;
 
 int mlv_fdz_work_bytes(int nx, int nz, int dtype, size_t* bytes);
@@ -301,6 +306,18 @@ int mlv_fdz_batch_graph_step(mlv_fdz_batch* batch, int nsteps, void* stream);
 /* Set one job's dt (and its implicit operators) for the following steps
  * (Integrator.override_dt, Integrator.py:46-48). */
 int mlv_fdz_batch_set_dt(mlv_fdz_batch* batch, int job, double dt, void* stream);
+/* As mlv_batch_run, for FD-z batches: the CFL ticker on the device from the
+ * physical ux, uz of each check loop's starting state (as
+ * rayleigh_benard_convection.py's Simulation.end_loop sees them); a changed
+ * dt also rebuilds the job's theta operators on the device. */
+int mlv_fdz_batch_run(mlv_fdz_batch* batch, int nsteps, int cadence, double cutoff, int graph,
+                      void* stream);
+/* As mlv_batch_job_time and mlv_batch_tracker, for FD-z batches (the
+ * scalar flux column is mean(T uz)). */
+int mlv_fdz_batch_job_time(mlv_fdz_batch* batch, int job, double* dt, double* t,
+                           long long* loops, int* breach);
+int mlv_fdz_batch_tracker(mlv_fdz_batch* batch, int job, double* out, int max_records,
+                          int* n_records);
 long long mlv_fdz_batch_steps(const mlv_fdz_batch* batch);
 /* Kernel timing (measurement only): milliseconds and launch counts of the
  * column (x transform) and row (z solve / update) kernels. */

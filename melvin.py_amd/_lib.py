@@ -42,7 +42,8 @@ EXPORTS = (
     "mlv_fdz_work_bytes", "mlv_fdz_batch_create", "mlv_fdz_batch_destroy",
     "mlv_fdz_batch_prime", "mlv_fdz_batch_step", "mlv_fdz_batch_graph_step",
     "mlv_fdz_batch_set_dt", "mlv_fdz_batch_steps", "mlv_fdz_batch_profile",
-    "mlv_fdz_batch_profile_read",
+    "mlv_fdz_batch_profile_read", "mlv_fdz_batch_run", "mlv_fdz_batch_job_time",
+    "mlv_fdz_batch_tracker",
 )
 
 
@@ -102,6 +103,8 @@ class FdzJobDesc(ctypes.Structure):
         ("phys", ctypes.c_void_p * 4),
         ("curr", ctypes.c_int),
         ("parity", ctypes.c_int),
+        ("t", ctypes.c_double),
+        ("loops", ctypes.c_longlong),
     ]
 
 
@@ -155,6 +158,10 @@ def _declare(lib):
         "mlv_fdz_batch_profile": ([vp, i], i),
         "mlv_fdz_batch_profile_read": ([vp, ctypes.POINTER(d),
                                         ctypes.POINTER(ctypes.c_longlong)], i),
+        "mlv_fdz_batch_run": ([vp, i, i, d, i, vp], i),
+        "mlv_fdz_batch_job_time": ([vp, i, ctypes.POINTER(d), ctypes.POINTER(d),
+                                    ctypes.POINTER(ctypes.c_longlong), ip], i),
+        "mlv_fdz_batch_tracker": ([vp, i, ctypes.POINTER(d), i, ip], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)

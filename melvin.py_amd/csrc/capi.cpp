@@ -117,6 +117,28 @@ int global_partials(int device, double** out) {
   return MLV_OK;
 }
 
+int copy_tracker(const double* hdr, double* out, int max_records, int* nrec) {
+  double h[kDiagHdr];
+  HIP_TRY(hipMemcpy(h, hdr, sizeof(h), hipMemcpyDeviceToHost), "read diagnostics");
+  const long long total = (long long)h[0];
+  const long long kept = total < kDiagCap ? total : kDiagCap;
+  const long long n = kept < max_records ? kept : max_records;
+  if (n > 0 && out) {
+    // oldest kept record first
+    std::vector<double> all((size_t)kDiagRec * kDiagCap);
+    HIP_TRY(hipMemcpy(all.data(), hdr + kDiagHdr, all.size() * sizeof(double),
+                      hipMemcpyDeviceToHost),
+            "read tracker");
+    for (long long i = 0; i < n; ++i) {
+      const long long r = (total - kept + i) % kDiagCap;
+      std::memcpy(out + (size_t)kDiagRec * i, all.data() + (size_t)kDiagRec * r,
+                  kDiagRec * sizeof(double));
+    }
+  }
+  *nrec = (int)n;
+  return MLV_OK;
+}
+
 }  // namespace abi
 }  // namespace mlv
 
@@ -139,7 +161,7 @@ struct mlv_plan {
 extern "C" {
 
 const char* mlv_last_error(void) { return g_err.c_str(); }
-int mlv_abi_version(void) { return 1; }
+int mlv_abi_version(void) { return 2; }
 
 int mlv_plan_create(mlv_plan** out, int nx, int nz, double lx, double lz, int disc, int dtype,
                     int fd_order) {
@@ -1045,25 +1067,7 @@ int mlv_batch_tracker(mlv_batch* bp, int job, double* out, int max_records, int*
   BatchHost* b = reinterpret_cast<BatchHost*>(bp);
   if (job < 0 || job >= b->njobs) return fail(MLV_ERR_ARG, "job index out of range");
   HIP_TRY(hipDeviceSynchronize(), "synchronize");
-  const double* base = b->diag[job] + (size_t)kDiagPart * b->nbB[job];
-  double h[4];
-  HIP_TRY(hipMemcpy(h, base, sizeof(h), hipMemcpyDeviceToHost), "read diagnostics");
-  const long long total = (long long)h[0];
-  const long long kept = total < kDiagCap ? total : kDiagCap;
-  const long long n = kept < max_records ? kept : max_records;
-  // oldest kept record first
-  std::vector<double> all((size_t)kDiagRec * kDiagCap);
-  if (n > 0 && out) {
-    HIP_TRY(hipMemcpy(all.data(), base + 4, all.size() * sizeof(double), hipMemcpyDeviceToHost),
-            "read tracker");
-    for (long long i = 0; i < n; ++i) {
-      const long long r = (total - kept + i) % kDiagCap;
-      std::memcpy(out + (size_t)kDiagRec * i, all.data() + (size_t)kDiagRec * r,
-                  kDiagRec * sizeof(double));
-    }
-  }
-  *nrec = (int)n;
-  return MLV_OK;
+  return copy_tracker(b->diag[job] + (size_t)kDiagPart * b->nbB[job], out, max_records, nrec);
 }
 
 int mlv_batch_set_dt(mlv_batch* bp, int job, double dt, void* stream) {

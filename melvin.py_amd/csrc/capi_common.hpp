@@ -17,6 +17,10 @@ int ilog2_exact(int n);
 int get_twiddles(int device, int dtype, const void** out);
 size_t max_lds_per_block(int device);
 size_t cplx_bytes(int dtype);
+// Copy up to max_records tracker records, oldest first, from the device
+// diagnostics header at hdr (diag.hpp layout) to out; the device must be
+// idle.  *nrec receives the count.
+int copy_tracker(const double* hdr, double* out, int max_records, int* nrec);
 
 }  // namespace abi
 }  // namespace mlv

@@ -37,7 +37,11 @@ struct FdzHost {
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
-  int graph_steps = 0;
+  int graph_steps = 0, graph_cadence = 0;
+  double graph_cutoff = 0.0;
+  int max_nn = 0;
+  std::vector<double*> diag;    // per job: partials, header, tracker records
+  std::vector<int> nbC;         // per job: column blocks of phase 0
   bool prof = false;
   std::vector<hipEvent_t> events;
   size_t used = 0;
@@ -136,6 +140,8 @@ int build_fdz_job(const mlv_fdz_job_desc& d, FdzJob<T>& J, int j) {
   J.lo[0] = idz2;
   J.lo[1] = -(d.alpha * d.dt * d.Pr) * idz2;
   J.lo[2] = -(d.alpha * d.dt * 1.0) * idz2;
+  J.t = d.t;
+  J.loops = d.loops;
   return MLV_OK;
 }
 
@@ -189,6 +195,8 @@ int create_impl(FdzHost* b, const mlv_fdz_job_desc* jobs, int njobs) {
     const int rb2 = (2 * J.nn + kFdzWaves - 1) / kFdzWaves;
     const int rb1 = (J.nn + kFdzWaves - 1) / kFdzWaves;
     for (int p = 0; p < 2; ++p) pcol[p][j + 1] = pcol[p][j] + (p < nph ? cb : 0);
+    J.nbC = cb;
+    b->max_nn = std::max(b->max_nn, J.nn);
     for (int p = 0; p < 3; ++p) {
       const int nb = p == 2 ? rb1 : (p < nph ? rb2 : 0);
       for (int v = 0; v < kFdzVariants; ++v) prow[p][v][j + 1] = prow[p][v][j] + (v == var ? nb : 0);
@@ -209,6 +217,18 @@ int create_impl(FdzHost* b, const mlv_fdz_job_desc* jobs, int njobs) {
     HIP_TRY(hipMemcpy(*dst, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice), "upload");
     return MLV_OK;
   };
+  // diagnostics: partials of phase 0's column blocks, header, records
+  b->diag.assign(njobs, nullptr);
+  b->nbC.assign(njobs, 0);
+  for (int j = 0; j < njobs; ++j) {
+    FdzJob<T>& J = host_job<T>(b, j);
+    const size_t bytes = diag_doubles(J.nbC) * sizeof(double);
+    int r = dalloc(bytes, (void**)&J.diag);
+    if (r) return r;
+    HIP_TRY(hipMemset(J.diag, 0, bytes), "clear diagnostics");
+    b->diag[j] = J.diag;
+    b->nbC[j] = J.nbC;
+  }
   // factor tables, chunk-major, one (cp, 1/den) pair per element
   for (int j = 0; j < njobs; ++j) {
     FdzJob<T>& J = host_job<T>(b, j);
@@ -243,17 +263,27 @@ int create_impl(FdzHost* b, const mlv_fdz_job_desc* jobs, int njobs) {
   return MLV_OK;
 }
 
+// Steps first .. first+nsteps-1 (counted from the batch's creation).  With
+// cadence > 0 the steps whose count is a multiple of cadence (loops 1,
+// 1+cadence, ... of Simulation.end_loop's CFL ticker) also produce the
+// diagnostics partials and are followed by the on-device CFL update.
 template <typename T>
 hipError_t enqueue(FdzHost* b, int nsteps, int write_phys, long long first, hipStream_t s,
-                   bool timed) {
-  const FdzJob<T>* jd = (const FdzJob<T>*)b->jobs_dev;
+                   bool timed, int cadence, double cutoff) {
+  FdzJob<T>* jd = (FdzJob<T>*)b->jobs_dev;
   const cx<T>* tw = (const cx<T>*)b->tw;
   for (int k = 0; k < nsteps; ++k) {
-    const int wp = (write_phys && k == nsteps - 1) ? 1 : 0;
+    const int wp = (write_phys && k == nsteps - 1) ? kFdzWritePhys : 0;
+    const bool dg = cadence > 0 && ((first + k) % cadence) == 0;
+    const int flags0 = wp | (dg ? kFdzDiag : 0);
     const int step = (int)((first + k) & 3);
     for (int ph = 0; ph < b->nphase; ++ph) {
       hipEvent_t* ev = timed ? prof_slots(b) : nullptr;
-      hipError_t e = fdz_phase<T>(jd, b->fl, ph, ph == 0 ? wp : 0, step, tw, s, ev);
+      hipError_t e = fdz_phase<T>(jd, b->fl, ph, ph == 0 ? flags0 : 0, step, tw, s, ev);
+      if (e != hipSuccess) return e;
+    }
+    if (dg) {
+      hipError_t e = fdz_diag<T>(jd, b->njobs, b->max_nn, cutoff, s);
       if (e != hipSuccess) return e;
     }
   }
@@ -274,9 +304,65 @@ hipError_t set_dt_impl(FdzHost* b, int j, double dt, hipStream_t s) {
 }
 
 hipError_t enqueue_steps(FdzHost* b, int nsteps, int write_phys, long long first, hipStream_t s,
-                         bool timed) {
-  return b->dtype == MLV_F64 ? enqueue<double>(b, nsteps, write_phys, first, s, timed)
-                             : enqueue<float>(b, nsteps, write_phys, first, s, timed);
+                         bool timed, int cadence = 0, double cutoff = 0.5) {
+  return b->dtype == MLV_F64
+             ? enqueue<double>(b, nsteps, write_phys, first, s, timed, cadence, cutoff)
+             : enqueue<float>(b, nsteps, write_phys, first, s, timed, cadence, cutoff);
+}
+
+int prime(FdzHost* b, hipStream_t s);
+
+int graph_run(FdzHost* b, int nsteps, int cadence, double cutoff, void* stream) {
+  if (nsteps <= 0 || (nsteps & 3)) return fail(MLV_ERR_ARG, "graph steps must be a positive multiple of 4");
+  if (b->steps & 3)
+    return fail(MLV_ERR_ARG, "graph replay needs the batch's step count at a multiple of 4");
+  if (cadence > 0 && ((nsteps % cadence) || (b->steps % cadence)))
+    return fail(MLV_ERR_ARG, "graph runs with CFL checks need nsteps and the step count at "
+                             "multiples of the cadence");
+  if (!b->primed) {
+    int rc = prime(b, (hipStream_t)stream);
+    if (rc) return rc;
+  }
+  if (!b->graph_exec || b->graph_steps != nsteps || b->graph_cadence != cadence ||
+      b->graph_cutoff != cutoff) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graph_exec = nullptr;
+    b->graph = nullptr;
+    if (!b->cap_stream)
+      HIP_TRY(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking), "capture stream");
+    HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    hipError_t e = enqueue_steps(b, nsteps, 0, 0, b->cap_stream, false, cadence, cutoff);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->cap_stream, &g);
+    if (e != hipSuccess) return hip_fail(e, "capture");
+    if (e2 != hipSuccess) return hip_fail(e2, "end capture");
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0), "instantiate");
+    b->graph_steps = nsteps;
+    b->graph_cadence = cadence;
+    b->graph_cutoff = cutoff;
+  }
+  HIP_TRY(hipGraphLaunch(b->graph_exec, (hipStream_t)stream), "graph launch");
+  b->steps += nsteps;
+  return MLV_OK;
+}
+
+template <typename T>
+int read_fdz_job(FdzHost* b, int job, double* dt, double* t, long long* loops) {
+  FdzJob<T> J;
+  HIP_TRY(hipMemcpy(&J, (const char*)b->jobs_dev + job * sizeof(FdzJob<T>), sizeof(J),
+                    hipMemcpyDeviceToHost),
+          "read job");
+  if (dt) *dt = J.dt;
+  if (t) *t = J.t;
+  if (loops) *loops = J.loops;
+  // keep the host copy's dt (and theta bands) in step for set_dt
+  FdzJob<T>& H = host_job<T>(b, job);
+  H.dt = J.dt;
+  H.lo[1] = J.lo[1];
+  H.lo[2] = J.lo[2];
+  return MLV_OK;
 }
 
 int prime(FdzHost* b, hipStream_t s) {
@@ -362,34 +448,51 @@ int mlv_fdz_batch_step(mlv_fdz_batch* bp, int nsteps, int write_phys, void* stre
 
 int mlv_fdz_batch_graph_step(mlv_fdz_batch* bp, int nsteps, void* stream) {
   if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
+  return graph_run(reinterpret_cast<FdzHost*>(bp), nsteps, 0, 0.0, stream);
+}
+
+int mlv_fdz_batch_run(mlv_fdz_batch* bp, int nsteps, int cadence, double cutoff, int graph,
+                      void* stream) {
+  if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
+  if (nsteps < 0 || cadence < 0 || !(cutoff > 0)) return fail(MLV_ERR_ARG, "bad run arguments");
   FdzHost* b = reinterpret_cast<FdzHost*>(bp);
-  if (nsteps <= 0 || (nsteps & 3)) return fail(MLV_ERR_ARG, "graph steps must be a positive multiple of 4");
-  if (b->steps & 3)
-    return fail(MLV_ERR_ARG, "graph replay needs the batch's step count at a multiple of 4");
+  if (graph) return graph_run(b, nsteps, cadence, cutoff, stream);
   if (!b->primed) {
     int rc = prime(b, (hipStream_t)stream);
     if (rc) return rc;
   }
-  if (!b->graph_exec || b->graph_steps != nsteps) {
-    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->graph_exec = nullptr;
-    b->graph = nullptr;
-    if (!b->cap_stream)
-      HIP_TRY(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking), "capture stream");
-    HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal), "begin capture");
-    hipError_t e = enqueue_steps(b, nsteps, 0, 0, b->cap_stream, false);
-    hipGraph_t g = nullptr;
-    hipError_t e2 = hipStreamEndCapture(b->cap_stream, &g);
-    if (e != hipSuccess) return hip_fail(e, "capture");
-    if (e2 != hipSuccess) return hip_fail(e2, "end capture");
-    b->graph = g;
-    HIP_TRY(hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0), "instantiate");
-    b->graph_steps = nsteps;
-  }
-  HIP_TRY(hipGraphLaunch(b->graph_exec, (hipStream_t)stream), "graph launch");
+  hipError_t e =
+      enqueue_steps(b, nsteps, 0, b->steps, (hipStream_t)stream, b->prof, cadence, cutoff);
+  if (e != hipSuccess) return hip_fail(e, "fdz run");
   b->steps += nsteps;
   return MLV_OK;
+}
+
+int mlv_fdz_batch_job_time(mlv_fdz_batch* bp, int job, double* dt, double* t, long long* loops,
+                           int* breach) {
+  if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
+  FdzHost* b = reinterpret_cast<FdzHost*>(bp);
+  if (job < 0 || job >= b->njobs) return fail(MLV_ERR_ARG, "job index out of range");
+  HIP_TRY(hipDeviceSynchronize(), "synchronize");
+  int rc = b->dtype == MLV_F64 ? read_fdz_job<double>(b, job, dt, t, loops)
+                               : read_fdz_job<float>(b, job, dt, t, loops);
+  if (rc) return rc;
+  if (breach) {
+    double h[2];
+    HIP_TRY(hipMemcpy(h, b->diag[job] + (size_t)kDiagPart * b->nbC[job], sizeof(h),
+                      hipMemcpyDeviceToHost),
+            "read diagnostics");
+    *breach = h[1] != 0.0;
+  }
+  return MLV_OK;
+}
+
+int mlv_fdz_batch_tracker(mlv_fdz_batch* bp, int job, double* out, int max_records, int* nrec) {
+  if (!bp || !nrec) return fail(MLV_ERR_ARG, "NULL argument");
+  FdzHost* b = reinterpret_cast<FdzHost*>(bp);
+  if (job < 0 || job >= b->njobs) return fail(MLV_ERR_ARG, "job index out of range");
+  HIP_TRY(hipDeviceSynchronize(), "synchronize");
+  return copy_tracker(b->diag[job] + (size_t)kDiagPart * b->nbC[job], out, max_records, nrec);
 }
 
 int mlv_fdz_batch_set_dt(mlv_fdz_batch* bp, int job, double dt, void* stream) {

@@ -1,6 +1,7 @@
 // Fused Fourier-x / FD-z step kernels over a job table (see fdz.hpp).
 #include <math.h>
 
+#include "diag_dev.hpp"
 #include "fdz.hpp"
 #include "grid.hpp"
 
@@ -363,6 +364,13 @@ __global__ void __launch_bounds__(64 * kFdzWaves)
     stage = J.scheme == MLV_FDZ_PC ? (phase == 0 ? FDZ_PRED : FDZ_CORR) : FDZ_MAIN;
     nwf = 2;
   }
+  if (phase == (J.scheme == MLV_FDZ_PC ? 1 : 0) && lb == pref[jb] && threadIdx.x == 0) {
+    // t += dt after each loop (Simulation.py:199-209); nothing else in this
+    // launch reads these two fields
+    FdzJob<T>* Jw = const_cast<FdzJob<T>*>(jobs) + jb;
+    Jw->t += J.dt;
+    Jw->loops += 1;
+  }
   if (lw >= nwf * J.nn || J.kt != KT) return;
   const int n = lw / nwf, f = lw % nwf;
   double2* buf = reinterpret_cast<double2*>(smem_raw) + (size_t)(2 * wave) * zstride;
@@ -396,7 +404,7 @@ __host__ __device__ constexpr int fdz_tc(int slots) {
 }
 
 template <typename T, int LOGN>
-__device__ __forceinline__ void fdz_cols_body(const FdzJob<T>& J, int local, int phase, int wphys,
+__device__ __forceinline__ void fdz_cols_body(const FdzJob<T>& J, int local, int phase, int flags,
                                               int step, const cx<T>* __restrict__ tw,
                                               cx<T>* lds) {
   constexpr int N = 1 << LOGN;
@@ -465,11 +473,32 @@ __device__ __forceinline__ void fdz_cols_body(const FdzJob<T>& J, int local, int
     }
     __syncthreads();
     fft_lds<T, LOGN, +1, kNT, P>(lds, nslot, tw);
-    const bool wp = wphys && !corr;
+    const bool wp = (flags & kFdzWritePhys) && !corr;
+    const bool dg = (flags & kFdzDiag) && !corr;
+    T mux = -INFINITY, muz = -INFINITY;  // NaN-propagating maxima, as np.max
+    double ke = 0.0, ens = 0.0, flux = 0.0;
     for (int e = threadIdx.x; e < npair * N; e += kNT) {
       const int pr = e / N, x = e % N;
       cx<T>* z = lds + (4 * pr) * LDSN + fft_pad(x);
       const cx<T> vx = z[0], vz = z[LDSN], w = z[2 * LDSN], t = z[3 * LDSN];
+      if (dg) {
+        // the imaginary parts belong to column 2 pr + 1 when it exists
+        const int c0 = 2 * pr;
+        const double t0 = (double)(t.x + tmean[c0]);
+        mux = nan_max(mux, vx.x);
+        muz = nan_max(muz, vz.x);
+        ke += (double)vz.x * (double)vz.x + (double)vx.x * (double)vx.x;
+        ens += (double)w.x * (double)w.x;
+        flux += t0 * (double)vz.x;
+        if (c0 + 1 < ncol) {
+          const double t1 = (double)(t.y + tmean[c0 + 1]);
+          mux = nan_max(mux, vx.y);
+          muz = nan_max(muz, vz.y);
+          ke += (double)vz.y * (double)vz.y + (double)vx.y * (double)vx.y;
+          ens += (double)w.y * (double)w.y;
+          flux += t1 * (double)vz.y;
+        }
+      }
       // real parts: column 2 pr, imaginary parts: column 2 pr + 1
       if (wp) {
         const int c0 = 2 * pr;
@@ -488,6 +517,7 @@ __device__ __forceinline__ void fdz_cols_body(const FdzJob<T>& J, int local, int
       z[2 * LDSN] = mk<T>(vx.x * t.x, vx.y * t.y);       // ux T'
       z[3 * LDSN] = mk<T>(vz.x * t.x, vz.y * t.y);       // uz T'
     }
+    if (dg) block_diag<kNT>((double)mux, (double)muz, ke, ens, flux, J.diag + (size_t)kDiagPart * local);
     __syncthreads();
     fft_lds<T, LOGN, -1, kNT, P>(lds, nslot, tw);
     const T hs = (T)(0.5 / (double)N);
@@ -525,7 +555,7 @@ __device__ __forceinline__ void fdz_cols_body(const FdzJob<T>& J, int local, int
 template <typename T>
 __global__ void __launch_bounds__(kNT)
     k_fdz_cols(const FdzJob<T>* __restrict__ jobs, const int* __restrict__ pref, int njobs,
-               int phase, int wphys, int step, const cx<T>* __restrict__ tw) {
+               int phase, int flags, int step, const cx<T>* __restrict__ tw) {
   extern __shared__ char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int lb = xcd_logical(pref[njobs]);
@@ -536,7 +566,7 @@ __global__ void __launch_bounds__(kNT)
   switch (J.lognx) {
 #define X(L)                                                          \
   case L:                                                             \
-    fdz_cols_body<T, L>(J, local, phase, wphys, step, tw, lds);       \
+    fdz_cols_body<T, L>(J, local, phase, flags, step, tw, lds);       \
     break;
     X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 #undef X
@@ -550,10 +580,8 @@ __global__ void __launch_bounds__(kNT)
 //              (LaplacianSolver.py:21-52)
 //   which > 0: theta diffusion, [-a/dz^2, 1 + a (kx^2 + 2/dz^2), -a/dz^2]
 // First and last rows are identity rows.
-__global__ void k_fdz_factors(double2* __restrict__ fac, int nn, int nz, int K, double fxabs,
-                              double dz, int which, double a) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nn) return;
+__device__ void fdz_factor_row(double2* __restrict__ fac, int n, int nz, int K, double fxabs,
+                               double dz, int which, double a) {
   double2* F = fac + (size_t)n * K * 64;
   const double k = (double)n * fxabs;
   const double k2 = k * k;
@@ -588,6 +616,45 @@ __global__ void k_fdz_factors(double2* __restrict__ fac, int nn, int nz, int K, 
   }
   // last row: identity (lower 0, diagonal 1), cp = 0
   F[((nz - 1) % K) * 64 + (nz - 1) / K] = make_double2(0.0, 1.0);
+}
+
+__global__ void k_fdz_factors(double2* __restrict__ fac, int nn, int nz, int K, double fxabs,
+                              double dz, int which, double a) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < nn) fdz_factor_row(fac, n, nz, K, fxabs, dz, which, a);
+}
+
+// One block per job: tracker record and CFL rule (diag_apply) from the
+// column kernel's partials; a changed dt also moves the theta operators'
+// interior bands and flags the job for k_fdz_refactor.
+template <typename T>
+__global__ void __launch_bounds__(kNT) k_fdz_diag(FdzJob<T>* jobs, double cutoff) {
+  FdzJob<T>& J = jobs[blockIdx.x];
+  double dt = J.dt;
+  const bool changed = diag_apply<kNT>(J.diag, J.nbC, J.dx, J.dz, (double)J.nx * (double)J.nz,
+                                       true, cutoff, J.t, J.loops, dt);
+  if (threadIdx.x == 0) {
+    J.refactor = changed ? 1 : 0;
+    if (changed) {
+      const double idz2 = 1.0 / (J.dz * J.dz);
+      J.dt = dt;
+      J.lo[1] = -(J.alpha * dt * J.Pr) * idz2;
+      J.lo[2] = -(J.alpha * dt * 1.0) * idz2;
+    }
+  }
+}
+
+// Theta tables of the jobs whose dt the last CFL update changed: grid
+// (modes / 64, jobs, 2 operators).
+template <typename T>
+__global__ void k_fdz_refactor(const FdzJob<T>* __restrict__ jobs) {
+  const FdzJob<T>& J = jobs[blockIdx.y];
+  if (!J.refactor || J.scheme == MLV_FDZ_EXPLICIT) return;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= J.nn) return;
+  const int which = 1 + (int)blockIdx.z;
+  const double a = J.alpha * J.dt * (which == 1 ? J.Pr : 1.0);
+  fdz_factor_row(J.fac[which], n, J.nz, J.K, J.fx, J.dz, which, a);
 }
 
 template <typename T>
@@ -626,13 +693,13 @@ static void launch_rows_all(const FdzJob<T>* jobs, const FdzLaunch& fl, int phas
 }
 
 template <typename T>
-hipError_t fdz_phase(const FdzJob<T>* jobs, const FdzLaunch& fl, int phase, int wphys, int step,
+hipError_t fdz_phase(const FdzJob<T>* jobs, const FdzLaunch& fl, int phase, int flags, int step,
                      const cx<T>* tw, hipStream_t s, hipEvent_t* ev) {
   if (ev) (void)hipEventRecord(ev[0], s);
   if (fl.ncol[phase] > 0) {
     MLV_SET_LDS_ONCE(k_fdz_cols<T>, fl.ldscol);
     hipLaunchKernelGGL(k_fdz_cols<T>, dim3(xcd_grid(fl.ncol[phase])), dim3(kNT), fl.ldscol, s,
-                       jobs, (const int*)fl.prefcol[phase], fl.njobs, phase, wphys, step, tw);
+                       jobs, (const int*)fl.prefcol[phase], fl.njobs, phase, flags, step, tw);
   }
   if (ev) (void)hipEventRecord(ev[1], s);
   launch_rows_all<T>(jobs, fl, phase, step, s);
@@ -661,6 +728,14 @@ hipError_t fdz_set_dt(FdzJob<T>* jobs, int job, double dt, double lo_w, double l
 }
 
 template <typename T>
+hipError_t fdz_diag(FdzJob<T>* jobs, int njobs, int max_nn, double cutoff, hipStream_t s) {
+  hipLaunchKernelGGL(k_fdz_diag<T>, dim3(njobs), dim3(kNT), 0, s, jobs, cutoff);
+  hipLaunchKernelGGL(k_fdz_refactor<T>, dim3((max_nn + 63) / 64, njobs, 2), dim3(64), 0, s,
+                     (const FdzJob<T>*)jobs);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t fdz_prepare(const FdzLaunch& fl) {
   if (fl.ldscol) MLV_SET_LDS_ONCE(k_fdz_cols<T>, fl.ldscol);
   if (fl.ldsrow) {
@@ -682,6 +757,8 @@ template hipError_t fdz_prologue<float>(const FdzJob<float>*, const FdzLaunch&, 
 template hipError_t fdz_prologue<double>(const FdzJob<double>*, const FdzLaunch&, int, hipStream_t);
 template hipError_t fdz_set_dt<float>(FdzJob<float>*, int, double, double, double, hipStream_t);
 template hipError_t fdz_set_dt<double>(FdzJob<double>*, int, double, double, double, hipStream_t);
+template hipError_t fdz_diag<float>(FdzJob<float>*, int, int, double, hipStream_t);
+template hipError_t fdz_diag<double>(FdzJob<double>*, int, int, double, hipStream_t);
 template hipError_t fdz_prepare<float>(const FdzLaunch&);
 template hipError_t fdz_prepare<double>(const FdzLaunch&);
 

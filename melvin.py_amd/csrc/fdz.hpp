@@ -30,6 +30,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "../../include/melvin_hip.h"
+#include "diag.hpp"
 #include "transforms.hpp"
 
 namespace mlv {
@@ -57,7 +58,19 @@ struct FdzJob {
   cx<T>* qw[4];            // Qx_omega, Qz_omega, Qx_T, Qz_T
   T* phys[4];              // optional physical ux, uz, omega, T
   double2* fac[3];         // (cp, 1/den) per operator: Poisson, theta(omega), theta(T)
+  // device-side time keeping and diagnostics (diag.hpp): the row kernel of
+  // a step's last phase advances t and loops; diag holds the column kernel's
+  // per-block partials (nbC blocks of phase 0), the header and the tracker
+  // records; refactor is set by the CFL update when it changed dt (the
+  // theta tables are rebuilt on the device right after)
+  double t;
+  long long loops;
+  int nbC, refactor;
+  double* diag;
 };
+
+// flags of phase 0's column kernel: physical outputs, diagnostics partials
+constexpr int kFdzWritePhys = 1, kFdzDiag = 2;
 
 struct FdzLaunch {
   int njobs;
@@ -78,8 +91,13 @@ inline int fdz_cols_per_block(int lognx) {
 }
 
 template <typename T>
-hipError_t fdz_phase(const FdzJob<T>* jobs, const FdzLaunch& fl, int phase, int wphys, int step,
+hipError_t fdz_phase(const FdzJob<T>* jobs, const FdzLaunch& fl, int phase, int flags, int step,
                      const cx<T>* tw, hipStream_t s, hipEvent_t* ev = nullptr);
+// After a step whose phase-0 column kernel ran with kFdzDiag: each job's
+// tracker record and CFL rule (dt, theta bands), then the theta tables of
+// the jobs whose dt changed (max_nn = largest nn of the batch).
+template <typename T>
+hipError_t fdz_diag(FdzJob<T>* jobs, int njobs, int max_nn, double cutoff, hipStream_t s);
 template <typename T>
 hipError_t fdz_prologue(const FdzJob<T>* jobs, const FdzLaunch& fl, int step, hipStream_t s);
 // (Re)build the factor tables of one job: which = 0 Poisson, 1 theta(omega),

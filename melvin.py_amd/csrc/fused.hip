@@ -1,45 +1,9 @@
 // Fused fully-spectral step kernels over a job table (see fused.hpp).
 #include "fused.hpp"
+#include "diag_dev.hpp"
 #include "grid.hpp"
 
 namespace mlv {
-
-// NaN-propagating maximum (np.max semantics; fmax would drop a NaN).
-template <typename T>
-__device__ __forceinline__ T nan_max(T a, T b) {
-  return (a != a || b != b) ? T(NAN) : (a > b ? a : b);
-}
-
-// Block reduction of the z pass's diagnostics partials; thread 0 writes
-// (max ux, max uz, sum(ux^2 + uz^2)) to out.
-template <typename T>
-__device__ void block_diag(T mux, T muz, double ke, double* out) {
-  __shared__ double red[kNT / 64][3];
-  double a = (double)mux, b = (double)muz;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    a = nan_max(a, __shfl_xor(a, off, 64));
-    b = nan_max(b, __shfl_xor(b, off, 64));
-    ke += __shfl_xor(ke, off, 64);
-  }
-  const int wv = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    red[wv][0] = a;
-    red[wv][1] = b;
-    red[wv][2] = ke;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < kNT / 64; ++w) {
-      a = nan_max(a, red[w][0]);
-      b = nan_max(b, red[w][1]);
-      ke += red[w][2];
-    }
-    out[0] = a;
-    out[1] = b;
-    out[2] = ke;
-  }
-}
 
 // --------------------------------------------------------------------------
 // Kernel A: spectral prologue + inverse x pass
@@ -221,7 +185,7 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     fft_lds<T, LOGN, +1, kNT, P>(lds, nslot, tw);
     const bool diag = flags & kDiag;
     T mux = -INFINITY, muz = -INFINITY;  // NaN-propagating maxima, as np.max
-    double ke = 0.0;
+    double ke = 0.0, ens = 0.0, flux = 0.0;
     for (int n = threadIdx.x; n < N; n += kNT) {
       T v[RB * NIN];
 #pragma unroll
@@ -241,6 +205,9 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
             mux = nan_max(mux, ux);
             muz = nan_max(muz, uz);
             ke += (double)uz * (double)uz + (double)ux * (double)ux;
+            const double w = (double)v[r * NIN];
+            ens += w * w;
+            if constexpr (Pg::FLUX >= 0) flux += (double)v[r * NIN + Pg::FLUX] * (double)uz;
           }
         }
       }
@@ -264,7 +231,7 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
         }
       }
     }
-    if (diag) block_diag<T>(mux, muz, ke, J.diag + (size_t)kDiagPart * local);
+    if (diag) block_diag<kNT>((double)mux, (double)muz, ke, ens, flux, J.diag + (size_t)kDiagPart * local);
     __syncthreads();
     fft_lds<T, LOGN, -1, kNT, P>(lds, nrows * NEQ, tw);
     for (int o = wv; o < nrows * NEQ; o += kNT / 64) {
@@ -609,64 +576,17 @@ __global__ void k_set_dt(JobDev<T>* jobs, int j, double dt) {
   if (threadIdx.x == 0 && blockIdx.x == 0) jobs[j].dt = dt;
 }
 
-// One block per job: reduce the z pass's partials, append a tracker record
-// (loop, t, dt, kinetic energy 0.5 sum(ux^2+uz^2)/(nx nz) as utility.py:
-// 42-59, max ux, max uz, next dt) and apply the CFL rule of Integrator.py:
-// 35-44: cfl_dt = min(dx / max ux, dz / max uz) with Python's min (a NaN in
-// the second operand is ignored, in the first it is returned); a breach or a
-// NaN sets the breach flag and leaves dt, else dt shrinks by 0.9 until
-// dt <= cutoff * cfl_dt.
+// One block per job, after a step whose z pass wrote diagnostics partials:
+// tracker record and CFL rule (diag_apply).  The first advected scalar
+// exists in vorticity and DDC jobs of two or more fields.
 template <typename T>
 __global__ void __launch_bounds__(kNT) k_diag(JobDev<T>* jobs, double cutoff) {
   JobDev<T>& J = jobs[blockIdx.x];
-  __shared__ double red[kNT / 64][3];
-  double a = -INFINITY, b = -INFINITY, ke = 0.0;
-  for (int i = threadIdx.x; i < J.nbB; i += kNT) {
-    const double* p = J.diag + (size_t)kDiagPart * i;
-    a = nan_max(a, p[0]);
-    b = nan_max(b, p[1]);
-    ke += p[2];
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    a = nan_max(a, __shfl_xor(a, off, 64));
-    b = nan_max(b, __shfl_xor(b, off, 64));
-    ke += __shfl_xor(ke, off, 64);
-  }
-  const int wv = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    red[wv][0] = a;
-    red[wv][1] = b;
-    red[wv][2] = ke;
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  for (int w = 1; w < kNT / 64; ++w) {
-    a = nan_max(a, red[w][0]);
-    b = nan_max(b, red[w][1]);
-    ke += red[w][2];
-  }
-  const double c1 = J.dx / a, c2 = J.dz / b;
-  const double cfl = c2 < c1 ? c2 : c1;
   double dt = J.dt;
-  const double used = dt;
-  double* hdr = J.diag + (size_t)kDiagPart * J.nbB;
-  if (dt > cfl || cfl != cfl) {
-    hdr[1] = 1.0;  // breach
-  } else {
-    while (dt > cutoff * cfl) dt = dt * 0.9;
+  const bool flux = J.problem != MLV_PROB_MHD && J.nfield >= 2;
+  if (diag_apply<kNT>(J.diag, J.nbB, J.dx, J.dz, (double)J.nx * (double)J.nz, flux, cutoff, J.t,
+                      J.loops, dt))
     J.dt = dt;
-  }
-  const long long nrec = (long long)hdr[0];
-  double* rec = hdr + 4 + (size_t)kDiagRec * (nrec % kDiagCap);
-  rec[0] = (double)J.loops;
-  rec[1] = J.t;
-  rec[2] = used;
-  rec[3] = 0.5 * ke / ((double)J.nx * (double)J.nz);
-  rec[4] = a;
-  rec[5] = b;
-  rec[6] = J.dt;
-  hdr[0] = (double)(nrec + 1);
 }
 
 // --------------------------------------------------------------------------

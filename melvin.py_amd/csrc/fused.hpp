@@ -25,6 +25,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "../../include/melvin_hip.h"
+#include "diag.hpp"
 #include "transforms.hpp"
 
 namespace mlv {
@@ -88,13 +89,6 @@ struct JobDev {
   double* diag;
 };
 
-constexpr int kDiagPart = 4;   // max ux, max uz, sum(ux^2 + uz^2), unused
-constexpr int kDiagRec = 7;    // loop, t, dt of the step, KE, max ux, max uz, next dt
-constexpr int kDiagCap = 4096; // records kept per job (the oldest are overwritten)
-__host__ __device__ constexpr size_t diag_doubles(int nbB) {
-  return (size_t)kDiagPart * nbB + 4 + (size_t)kDiagRec * kDiagCap;
-}
-
 // bfloat16 pair <-> float2 (round to nearest even; the residuals are finite)
 __device__ __forceinline__ float2 bf2_load(uint32_t v) {
   return make_float2(__uint_as_float(v << 16), __uint_as_float(v & 0xffff0000u));
@@ -113,6 +107,7 @@ template <int NF> struct BVortProg {
   static constexpr int NIN = NF + 2;  // fields, ux, uz
   static constexpr int NEQ = NF;
   static constexpr int UX = NF, UZ = NF + 1;  // velocity channels
+  static constexpr int FLUX = NF >= 2 ? 1 : -1;  // scalar whose flux f*uz is tracked
   template <typename T>
   __device__ __forceinline__ static void products(const T* v, T* px, T* pz) {
 #pragma unroll
@@ -129,6 +124,7 @@ template <> struct BProg<BV_VORT3> : BVortProg<3> {};
 template <> struct BProg<BV_MHD1> {
   static constexpr int NIN = 6;
   static constexpr int UX = 2, UZ = 3;
+  static constexpr int FLUX = -1;
   static constexpr int NEQ = 1;
   template <typename T>
   __device__ __forceinline__ static void products(const T* v, T* px, T* pz) {
@@ -139,6 +135,7 @@ template <> struct BProg<BV_MHD1> {
 template <> struct BProg<BV_MHD2> {
   static constexpr int NIN = 6;
   static constexpr int UX = 2, UZ = 3;
+  static constexpr int FLUX = -1;
   static constexpr int NEQ = 1;
   template <typename T>
   __device__ __forceinline__ static void products(const T* v, T* px, T* pz) {

@@ -31,7 +31,7 @@ import math
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, fused
 
 SCHEMES = {
     "explicit": _lib.FDZ_EXPLICIT,
@@ -159,11 +159,17 @@ class FdzJob:
             for i, p in enumerate(self.phys):
                 d.phys[i] = p.data_ptr()
         d.curr, d.parity = self.curr, self.parity
+        d.t, d.loops = self.t, self.steps
         return d
 
 
-class FdzBatch:
-    """A set of FD-z jobs advanced together; all jobs share one precision."""
+class FdzBatch(fused.DeviceRun):
+    """A set of FD-z jobs advanced together; all jobs share one precision.
+    ``run_device`` / ``sync`` / ``tracker`` / ``save_trackers`` run the CFL
+    ticker and diagnostics on the device (mlv_fdz_batch_run); the tracker's
+    scalar flux is mean(T uz)."""
+
+    _C = "mlv_fdz_batch"
 
     def __init__(self, jobs):
         if not jobs:
@@ -259,6 +265,11 @@ def cfl_dt(job, dt, cutoff=0.5):
 def run(batch, nsteps, cfl_cadence=10, cutoff=0.5, on_cfl=None):
     """``fused.run`` for FD-z batches: the reference's CFL ticker around
     the fused steps."""
-    from . import fused
-
     return fused.run(batch, nsteps, cfl_cadence, cutoff, on_cfl, cfl=cfl_dt)
+
+
+def run_device(batch, nsteps, cfl_cadence=10, cutoff=0.5, chunk=None):
+    """``fused.run_device`` for FD-z batches: the same CFL ticker evaluated
+    on the device (a changed dt rebuilds the theta operators there too),
+    captured graphs of whole cadence periods, one synchronisation."""
+    return fused.run_device(batch, nsteps, cfl_cadence, cutoff, chunk)

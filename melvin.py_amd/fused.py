@@ -187,11 +187,90 @@ class Job:
         return d
 
 
-# Columns of the tracker records written by the on-device CFL check.
-TRACKER_COLUMNS = ("loop", "t", "dt", "kinetic_energy", "max_ux", "max_uz", "next_dt")
+# Columns of the tracker records written by the on-device CFL check:
+# kinetic energy as utility.py:42-59, enstrophy 0.5 mean(w^2), and
+# scalar_flux = mean(f uz) of the job's first advected scalar (NaN without
+# one), whose complement 1 - flux is double_diffusive_convection.py:20-23's
+# Nusselt number.
+TRACKER_COLUMNS = ("loop", "t", "dt", "kinetic_energy", "max_ux", "max_uz", "next_dt",
+                   "enstrophy", "scalar_flux")
 
 
-class Batch:
+class DeviceRun:
+    """On-device CFL ticker and tracker of a batch (the fully spectral
+    ``Batch`` and the FD-z ``FdzBatch``); ``_C`` is the C entry-point prefix."""
+
+    _C = "mlv_batch"
+
+    def run_device(self, nsteps, cadence=10, cutoff=0.5, graph=False, stream=None):
+        """Queue ``nsteps`` loops with the CFL ticker applied on the device
+        (no host synchronisation; see mlv_batch_run and mlv_fdz_batch_run).
+        Host-side dt and t of the jobs are stale until ``sync()``."""
+        if nsteps <= 0:
+            return
+        if graph and (self.jobs[0].steps - self._steps0[0]) % 4:
+            raise ValueError("graph replay needs the batch step count at a multiple of 4")
+        _lib.call(self._C + "_run", self.handle, int(nsteps), int(cadence), float(cutoff),
+                  1 if graph else 0, _lib.stream_handle(stream))
+        for j in self.jobs:
+            j.steps += nsteps
+            j.curr = (j.curr + nsteps) % j.ab_order
+            j.parity = (j.parity + nsteps) & 1
+
+    def sync(self):
+        """Read every job's device-side dt and t; returns the indices of
+        jobs whose CFL check found a breach."""
+        breached = []
+        for i, j in enumerate(self.jobs):
+            dt, t = ctypes.c_double(), ctypes.c_double()
+            loops, br = ctypes.c_longlong(), ctypes.c_int()
+            _lib.call(self._C + "_job_time", self.handle, i, ctypes.byref(dt), ctypes.byref(t),
+                      ctypes.byref(loops), ctypes.byref(br))
+            j.dt, j.t = dt.value, t.value
+            if br.value:
+                breached.append(i)
+        return breached
+
+    def tracker(self, index, max_records=4096):
+        """The job's tracker records (oldest first) as a dict of arrays
+        named by TRACKER_COLUMNS; DDC jobs also get ``nusselt_number``."""
+        nc = len(TRACKER_COLUMNS)
+        buf = (ctypes.c_double * (nc * max_records))()
+        n = ctypes.c_int()
+        _lib.call(self._C + "_tracker", self.handle, int(index), buf, int(max_records),
+                  ctypes.byref(n))
+        a = np.frombuffer(buf, dtype=np.float64)[: nc * n.value].reshape(n.value, nc)
+        out = {name: a[:, k].copy() for k, name in enumerate(TRACKER_COLUMNS)}
+        if getattr(self.jobs[index], "problem", None) == "ddc":
+            out["nusselt_number"] = 1.0 - out["scalar_flux"]
+        return out
+
+    def save_trackers(self, index, directory=".", cadence=100):
+        """Write job ``index``'s tracker series as the reference's
+        ScalarTracker files (ScalarTracker.py:15-19: ``t`` and ``values``
+        arrays): kinetic_energy.npz, enstrophy.npz, and for DDC jobs
+        nusselt_number.npz (double_diffusive_convection.py:80-88).  The
+        reference's tracker ticker fires after loops 1, 1 + cadence, ...
+        (Ticker.py:14-25, tracker_cadence = 100 by default); the device
+        records those loops when cadence is a multiple of the CFL cadence
+        of the run, and only those rows are written."""
+        import os
+
+        tr = self.tracker(index)
+        keep = (tr["loop"].astype(np.int64) - 1) % int(cadence) == 0
+        tr = {k: v[keep] for k, v in tr.items()}
+        names = ["kinetic_energy", "enstrophy"]
+        if "nusselt_number" in tr:
+            names.append("nusselt_number")
+        paths = []
+        for name in names:
+            p = os.path.join(directory, name + ".npz")
+            np.savez(p, t=tr["t"], values=tr[name])
+            paths.append(p)
+        return paths
+
+
+class Batch(DeviceRun):
     """A set of jobs advanced together; all jobs share one precision."""
 
     def __init__(self, jobs):
@@ -240,45 +319,6 @@ class Batch:
         job.dt = float(dt)
         _lib.call("mlv_batch_set_dt", self.handle, int(index), float(dt),
                   _lib.stream_handle(stream))
-
-    def run_device(self, nsteps, cadence=10, cutoff=0.5, graph=False, stream=None):
-        """Queue ``nsteps`` loops with the CFL ticker applied on the device
-        (no host synchronisation; see mlv_batch_run).  Host-side dt and t of
-        the jobs are stale until ``sync()``."""
-        if nsteps <= 0:
-            return
-        if graph and (self.jobs[0].steps - self._steps0[0]) % 4:
-            raise ValueError("graph replay needs the batch step count at a multiple of 4")
-        _lib.call("mlv_batch_run", self.handle, int(nsteps), int(cadence), float(cutoff),
-                  1 if graph else 0, _lib.stream_handle(stream))
-        for j in self.jobs:
-            j.steps += nsteps
-            j.curr = (j.curr + nsteps) % j.ab_order
-            j.parity = (j.parity + nsteps) & 1
-
-    def sync(self):
-        """Read every job's device-side dt and t; returns the indices of
-        jobs whose CFL check found a breach."""
-        breached = []
-        for i, j in enumerate(self.jobs):
-            dt, t = ctypes.c_double(), ctypes.c_double()
-            loops, br = ctypes.c_longlong(), ctypes.c_int()
-            _lib.call("mlv_batch_job_time", self.handle, i, ctypes.byref(dt), ctypes.byref(t),
-                      ctypes.byref(loops), ctypes.byref(br))
-            j.dt, j.t = dt.value, t.value
-            if br.value:
-                breached.append(i)
-        return breached
-
-    def tracker(self, index, max_records=4096):
-        """The job's tracker records (oldest first) as a dict of arrays
-        named by TRACKER_COLUMNS."""
-        buf = (ctypes.c_double * (7 * max_records))()
-        n = ctypes.c_int()
-        _lib.call("mlv_batch_tracker", self.handle, int(index), buf, int(max_records),
-                  ctypes.byref(n))
-        a = np.frombuffer(buf, dtype=np.float64)[: 7 * n.value].reshape(n.value, 7)
-        return {name: a[:, k].copy() for k, name in enumerate(TRACKER_COLUMNS)}
 
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:

@@ -35,7 +35,7 @@ def test_library_exports_every_declared_symbol():
     for name in declared:
         assert hasattr(lib, name), name
     assert set(declared) == set(L.EXPORTS)
-    assert lib.mlv_abi_version() == 1
+    assert lib.mlv_abi_version() == 2
 
 
 def test_job_desc_layout_matches_c(tmp_path):
