@@ -819,15 +819,18 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
     ldsCn = std::max(ldsCn, ldsn);
   }
   if (!rc) {
-    // Kernel C runs wide (more columns per block) only when its grid covers
-    // the device's block slots twice over at two blocks per CU; a smaller
-    // grid is better spread with narrow blocks (measured on MI355X: KH
-    // 1024^2, 8 jobs narrow -10% time, 32 jobs wide -7%).
+    // Kernel C: single precision always runs narrow -- 16 points per thread,
+    // the stencil's neighbours read through the cache, 112 VGPRs, so four
+    // blocks (16 wavefronts) share a CU and their loads overlap the others'
+    // transforms (measured on MI355X, KH 1024^2: -7 % kernel time at 8 and
+    // at 32 jobs against the wide form).  Double precision needs the
+    // register file either way and runs wide when its grid covers the
+    // device's block slots twice over at two blocks per CU.
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess ||
         cus <= 0)
       cus = 256;
-    bl.wideC = pC[0][njobs] >= 4 * cus ? 1 : 0;
+    bl.wideC = (b->dtype == MLV_F64 && pC[0][njobs] >= 4 * cus) ? 1 : 0;
     if (!bl.wideC) {
       for (int s = 0; s < 2; ++s) pC[s] = pCn[s];
       bl.ldsC = ldsCn;

@@ -312,7 +312,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   const int order = J.fd_order;
   const T ihx = order == 2 ? (T)(1.0 / (2.0 * J.dx)) : (T)(1.0 / (3.0 * J.dx));
   constexpr int ITER = (N * TC) / kNT;
-  constexpr int U = 8;
+  constexpr int U = (sizeof(T) == 4 && !WIDE) ? 4 : 8;
   static_assert(ITER % U == 0, "load batching");
   // sz depends only on the column; every thread keeps one column (e % TC
   // is fixed because kNT is a multiple of TC)
@@ -321,7 +321,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   const bool cok = cme < ncol;
   const size_t col = (size_t)(c0 + cme);
   cx<T>* __restrict__ slot = lds + cme * LDSN;
-  if constexpr (sizeof(T) == 4) {
+  if constexpr (sizeof(T) == 4 && WIDE) {
     // 1. Px column into the slot
 #pragma unroll 1
     for (int i0 = 0; i0 < ITER; i0 += U) {
@@ -371,8 +371,9 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
       slot[fft_pad(x)] = dv[i];
     }
   } else {
-    // double precision: the register file is the limit, so the stencil
-    // reads its neighbours straight from global memory (L2 hits)
+    // double precision and the narrow policy: the register file is the
+    // limit, so the stencil reads its neighbours straight from global
+    // memory (L2 hits)
 #pragma unroll 1
     for (int i0 = 0; i0 < ITER; i0 += U) {
       cx<T> pa[U], pb[U], pz[U], pa2[U], pb2[U];
@@ -423,7 +424,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   // spectral state is kz-major, so a column is contiguous).  Loads of U2
   // row groups are issued before any is used.
   constexpr int KR = (2 * ((N - 1) / 3) + 1 + 63) / 64;  // bound on ceil(rows / 64)
-  constexpr int U2 = 4;
+  constexpr int U2 = WIDE ? 4 : 2;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const bool ddc = J.problem == MLV_PROB_DDC;
   uint32_t* __restrict__ carry = sizeof(T) == 4 ? J.comp[field] : nullptr;
@@ -539,10 +540,13 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   }
 }
 
+// Narrow single-precision blocks are held to 128 VGPRs (four wavefronts per
+// SIMD, matching the four blocks per CU their LDS allows).
 template <typename T, int WIDE>
-__global__ void __launch_bounds__(kNT) k_stepC(const JobDev<T>* __restrict__ jobs,
-                                               const int* __restrict__ pref, int njobs, int stage,
-                                               int step, const cx<T>* __restrict__ tw) {
+__global__ void __launch_bounds__(kNT)
+    __attribute__((amdgpu_waves_per_eu((WIDE || sizeof(T) == 8) ? 1 : 4)))
+    k_stepC(const JobDev<T>* __restrict__ jobs, const int* __restrict__ pref, int njobs,
+            int stage, int step, const cx<T>* __restrict__ tw) {
   extern __shared__ char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int lb = xcd_logical(pref[njobs]);

@@ -154,8 +154,8 @@ __host__ __device__ constexpr int fused_cols() {
 // of fft_points_col (32 points from N = 1024 in single precision: more
 // columns per block); wide = 0 keeps 16 points up to N = 1024 (half the LDS
 // per block, so twice the blocks per CU).  The x-inverse always runs narrow;
-// the x-forward/update kernel runs wide when its grid fills the device
-// several times over and narrow otherwise.
+// the x-forward/update kernel runs narrow in single precision and, in
+// double precision, wide when its grid fills the device several times over.
 template <typename T>
 __host__ __device__ constexpr int fused_points(int logn, int wide) {
   return wide ? fft_points_col<T>(logn) : (logn >= 11 ? 32 : 16);
