@@ -783,8 +783,18 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
     const int slw = f64 ? fused_cols_host<double>(lognx, 1) : fused_cols_host<float>(lognx, 1);
     const size_t ldsn = f64 ? fused_lds_bytes<double>(lognx, 0) : fused_lds_bytes<float>(lognx, 0);
     const size_t ldsw = f64 ? fused_lds_bytes<double>(lognx, 1) : fused_lds_bytes<float>(lognx, 1);
-    const int slz = f64 ? slots_row<double>(lognz) : slots_row<float>(lognz);
-    const size_t ldsz = f64 ? fft_lds_bytes_row<double>(lognz) : fft_lds_bytes_row<float>(lognz);
+    // z pass: points per thread by program (bpoints); the largest LDS any
+    // of the job's programs takes is checked here
+    auto zslots = [&](int v) {
+      const int p = f64 ? bpoints<double>(lognz, bvar_nin(v), bvar_neq(v))
+                        : bpoints<float>(lognz, bvar_nin(v), bvar_neq(v));
+      return (kNT * p) >> lognz;
+    };
+    auto zlds = [&](int v) {
+      return (size_t)zslots(v) * fft_ldsn(1 << lognz) * (f64 ? 16 : 8);
+    };
+    size_t ldsz = 0;
+    for (int s = 0; s < nstage; ++s) ldsz = std::max(ldsz, zlds(bvar[s]));
     if (ldsn > lds_max || ldsw > lds_max || ldsz > lds_max || sln < 1 || slw < 1) {
       rc = fail(MLV_ERR_UNSUPPORTED, "job " + std::to_string(j) + ": grid too large for LDS");
       break;
@@ -794,6 +804,7 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
       int a = 0, c = 0, cn = 0, bb = 0, bv = 0;
       if (s < nstage) {
         const int nin_v = bvar_nin(bvar[s]), neq_v = bvar_neq(bvar[s]);
+        const int slz = zslots(bvar[s]);
         if (!bfits(nin_v, neq_v, slz)) {
           rc = fail(MLV_ERR_UNSUPPORTED,
                     "job " + std::to_string(j) + ": problem does not fit the z-pass LDS at this nz");
@@ -805,7 +816,7 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
         cn = neq[s] * ((nm + sln - 1) / sln);
         bb = (nx + rb - 1) / rb;
         bv = bvar[s];
-        bl.ldsB[bv] = std::max(bl.ldsB[bv], ldsz);
+        bl.ldsB[bv] = std::max(bl.ldsB[bv], zlds(bv));
       }
       if (s == 0) b->nbB.push_back(bb);
       pA[s][j + 1] = pA[s][j] + a;

@@ -130,7 +130,7 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
   using Pg = BProg<V>;
   constexpr int NIN = Pg::NIN, NEQ = Pg::NEQ;
   constexpr int N = 1 << LOGN;
-  constexpr int P = fft_points_row<T>(LOGN);
+  constexpr int P = bpoints<T>(LOGN, NIN, NEQ);
   constexpr int SL = fft_slots<LOGN, kNT, P>();
   constexpr int LDSN = fft_ldsn(N);
   if constexpr (!bfits(NIN, NEQ, SL)) {
@@ -144,12 +144,17 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     const int nseq = nrows * NIN;
     const int nslot = (nseq + 1) / 2;
     cx<T>* __restrict__ work = J.work;
-    // One wavefront per slot: lane l handles k = l + 64 i, so no index
-    // needs a division by the runtime row length nm.
+    // One wavefront per slot (WPS per slot when the block has fewer slots
+    // than wavefronts, splitting its k range): lane l handles k = l + 64 i,
+    // so no index needs a division by the runtime row length nm.
     constexpr int KIT = ((N - 1) / 3 + 63) / 64;  // bound on ceil(nm / 64)
     constexpr int KB = KIT < 6 ? KIT : 6;
+    constexpr int NW = kNT / 64;
+    constexpr int SLN = (RB * NIN + 1) / 2;
+    constexpr int WPS = SLN >= NW ? 1 : NW / SLN;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int s = wv; s < nslot; s += kNT / 64) {
+    const int ssub = wv % WPS;
+    for (int s = wv / WPS; s < nslot; s += NW / WPS) {
       cx<T>* z = lds + s * LDSN;
       const int qa = 2 * s, qb = qa + 1;
       const cx<T>* __restrict__ pa = work + (size_t)(qa % NIN) * chs + (size_t)(x0 + qa / NIN) * nm;
@@ -157,7 +162,7 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
           qb < nseq ? work + (size_t)(qb % NIN) * chs + (size_t)(x0 + qb / NIN) * nm : nullptr;
       // loads of up to KB iterations in flight before any is used
 #pragma unroll 1
-      for (int i0 = 0; i0 < KIT; i0 += KB) {
+      for (int i0 = ssub * KB; i0 < KIT; i0 += WPS * KB) {
         cx<T> av[KB], bv[KB];
 #pragma unroll
         for (int i = 0; i < KB; ++i) {
@@ -179,7 +184,7 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
           }
         }
       }
-      for (int k = nm + lane; k <= N - nm; k += 64) z[fft_pad(k)] = czero<T>();
+      for (int k = nm + lane + 64 * ssub; k <= N - nm; k += 64 * WPS) z[fft_pad(k)] = czero<T>();
     }
     __syncthreads();
     fft_lds<T, LOGN, +1, kNT, P>(lds, nslot, tw);
@@ -234,13 +239,18 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     if (diag) block_diag<kNT>((double)mux, (double)muz, ke, ens, flux, J.diag + (size_t)kDiagPart * local);
     __syncthreads();
     fft_lds<T, LOGN, -1, kNT, P>(lds, nrows * NEQ, tw);
-    for (int o = wv; o < nrows * NEQ; o += kNT / 64) {
+    // one wavefront per output row (WPO per row when there are fewer rows
+    // than wavefronts)
+    constexpr int NO = RB * NEQ;
+    constexpr int WPO = NO >= NW ? 1 : NW / NO;
+    const int osub = wv % WPO;
+    for (int o = wv / WPO; o < nrows * NEQ; o += NW / WPO) {
       const int r = o / NEQ, eq = o % NEQ;
       const cx<T>* z = lds + o * LDSN;
       cx<T>* __restrict__ ox = work + (size_t)pchan(NIN, eq, 0) * chs + (size_t)(x0 + r) * nm;
       cx<T>* __restrict__ oz = work + (size_t)pchan(NIN, eq, 1) * chs + (size_t)(x0 + r) * nm;
 #pragma unroll 6
-      for (int i = 0; i < KIT; ++i) {
+      for (int i = osub; i < KIT; i += WPO) {
         const int k = lane + 64 * i;
         if (k < nm) {
           const cx<T> y = z[fft_pad(k)];
@@ -431,13 +441,18 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   const int prev1 = (cur + q - 1) % q, prev2 = (cur + q - 2) % q, prev3 = (cur + q - 3) % q;
   const float fcoeff = (float)coeff, ffx2 = (float)J.fx2, ffz2 = (float)J.fz2;
   const float fdt = (float)dt, fc2 = (float)c2;
+  // one wavefront per column (WPC per column when the block has fewer
+  // columns than wavefronts, splitting its row groups)
+  constexpr int NW = kNT / 64;
+  constexpr int WPC = TC >= NW ? 1 : NW / TC;
+  const int wsub = wv % WPC;
 #pragma unroll 1
-  for (int cc = wv; cc < ncol; cc += kNT / 64) {
+  for (int cc = wv / WPC; cc < ncol; cc += NW / WPC) {
     const int m = c0 + cc;
     const size_t cbase = (size_t)m * rows;
     const cx<T>* __restrict__ dcol = lds + cc * LDSN;
 #pragma unroll 1
-    for (int i0 = 0; i0 < KR; i0 += U2) {
+    for (int i0 = wsub * U2; i0 < KR; i0 += WPC * U2) {
       cx<T> vv[U2], h1[U2], h2[U2], h3[U2], lo[U2], e1[U2], e2[U2];
 #pragma unroll
       for (int u = 0; u < U2; ++u) {

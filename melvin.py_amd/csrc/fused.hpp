@@ -152,13 +152,15 @@ __host__ __device__ constexpr int fused_cols() {
 }
 // Points per thread of the x passes.  wide = 1 is the column-pass policy
 // of fft_points_col (32 points from N = 1024 in single precision: more
-// columns per block); wide = 0 keeps 16 points up to N = 1024 (half the LDS
-// per block, so twice the blocks per CU).  The x-inverse always runs narrow;
+// columns per block); wide = 0 keeps 16 points up to N = 2048 (half the LDS
+// per block, so twice the blocks per CU, and a batch that mixes 2048 with
+// smaller grids keeps the smaller grids' occupancy: the launch's LDS is the
+// largest job's).  The x-inverse always runs narrow;
 // the x-forward/update kernel runs narrow in single precision and, in
 // double precision, wide when its grid fills the device several times over.
 template <typename T>
 __host__ __device__ constexpr int fused_points(int logn, int wide) {
-  return wide ? fft_points_col<T>(logn) : (logn >= 11 ? 32 : 16);
+  return wide ? fft_points_col<T>(logn) : (logn >= 12 ? 32 : 16);
 }
 template <typename T>
 inline int fused_cols_host(int logn, int wide) {
@@ -184,6 +186,16 @@ __host__ __device__ constexpr int brows(int nin, int neq, int slots) {
 }
 __host__ __device__ constexpr bool bfits(int nin, int neq, int slots) {
   return ((nin + 1) / 2 <= slots) && (neq <= slots);
+}
+// Points per thread of the z pass of a program with nin inputs and neq
+// equations: 16 up to N = 1024, and at N = 2048 when the program fits the
+// two slots that leaves (for the reason given at fused_points); otherwise
+// the row policy's 32.
+template <typename T>
+__host__ __device__ constexpr int bpoints(int logn, int nin, int neq) {
+  return (logn <= 10 || (logn == 11 && bfits(nin, neq, (kNT * 16) >> 11)))
+             ? 16
+             : fft_points_row<T>(logn);
 }
 __host__ __device__ constexpr int bvar_nin(int v) { return v <= BV_VORT3 ? v + 3 : 6; }
 __host__ __device__ constexpr int bvar_neq(int v) { return v <= BV_VORT3 ? v + 1 : 1; }

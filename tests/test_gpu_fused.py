@@ -179,3 +179,56 @@ def test_graph_replays_match_eager_steps(mlv):
     for a, b_ in zip(res["eager"], res["graph"]):
         for x, y in zip(a, b_):
             np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("nx,nz", [(2048, 32), (32, 2048)])
+@pytest.mark.parametrize("problem", ["vort1", "vort2", "ddc", "mhd"])
+def test_every_program_at_2048_vs_oracle(mlv, problem, nx, nz):
+    """Each z-pass program at N = 2048 along x and along z (16 points per
+    thread where the program fits two LDS slots, 32 otherwise; fewer
+    columns or rows than wavefronts per block), 3 steps FP64 against the
+    oracle, and bit-identical when batched with a small job."""
+    params = dict(nx=nx, nz=nz, lx=2.0, lz=1.5, initial_dt=1e-5, Pr=7.0, R0=1.5, tau=0.01,
+                  Re=1e3, S=1e3)
+    p = O.make_params(params)
+    rng = np.random.default_rng(nx + nz)
+    f = [O.to_spectral(p, rng.standard_normal((nx, nz))) for _ in range(3)]
+    spec = {
+        "vort1": (dict(problem="vorticity", coeff=[1e-3]), ("w",),
+                  lambda p, st: O.step_vorticity(p, st, 1e-3)),
+        "vort2": (dict(problem="vorticity", coeff=[1e-3, 1e-3]), ("w", "c"),
+                  lambda p, st: O.step_vorticity(p, st, 1e-3, ("c",))),
+        "ddc": (dict(problem="ddc", coeff=[p.Pr, 1.0, p.tau], param=[p.Pr, p.R0]),
+                ("w", "tmp", "xi"), O.step_ddc),
+        "mhd": (dict(problem="mhd", coeff=[1 / p.Re, 1 / p.S]), ("w", "j"), O.step_mhd),
+    }
+    kw, names, body = spec[problem]
+    init = f[:len(names)]
+
+    def make():
+        k = dict(kw)
+        j = _job(mlv, k.pop("problem"), params, **k)
+        j.set_state(init)
+        return j
+
+    print(f"[fused] {problem} {nx}x{nz}", flush=True)
+    job = make()
+    b = mlv.fused.Batch([job])
+    b.step(3)
+    st = O.new_state(p, names, dt=1e-5)
+    for k, v in zip(names, init):
+        st[k] = v.copy()
+    for _ in range(3):
+        body(p, st)
+    for got, name in zip(job.state(), names):
+        assert rel(got, st[name]) < 1e-12, name
+    b.close()
+    ps = dict(nx=32, nz=32, lx=1.0, lz=1.0, initial_dt=1e-4)
+    small = _job(mlv, "vorticity", ps, coeff=[1e-3])
+    small.set_state([O.to_spectral(O.make_params(ps), rng.standard_normal((32, 32)))])
+    mixed = make()
+    b = mlv.fused.Batch([small, mixed])
+    b.step(3)
+    for x, y in zip(mixed.state(), job.state()):
+        np.testing.assert_array_equal(x, y)
+    b.close()
