@@ -214,9 +214,10 @@ int mlv_batch_graph_step(mlv_batch* batch, int nsteps, void* stream);
 /* Set one job's dt for the following steps (stream-ordered). */
 int mlv_batch_set_dt(mlv_batch* batch, int job, double dt, void* stream);
 /* nsteps loops with Simulation.end_loop's CFL ticker on the device
- * (Simulation.py:199-209, Ticker.py:14-25, Integrator.py:35-44): after the
- * loops whose 0-based count is a multiple of cadence, each job's maxima of
- * its physical ux, uz (FP64, NaN-propagating, as xp.max) give
+ * (Simulation.py:199-209, Ticker.py:14-25, Integrator.py:35-44): after each
+ * job's loops 1, 1+cadence, ... -- counted from the job descriptor's loops,
+ * so a resumed simulation keeps its phase -- the job's maxima of its
+ * physical ux, uz (FP64, NaN-propagating, as xp.max) give
  * cfl_dt = min(dx / max ux, dz / max uz); a breach or NaN raises the job's
  * breach flag, else dt shrinks by 0.9 until dt <= cutoff * cfl_dt.  No host
  * synchronisation; graph = 1 replays a captured graph (nsteps and the step

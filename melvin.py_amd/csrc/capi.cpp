@@ -550,6 +550,7 @@ struct BatchHost {
   std::vector<char> host;          // host copy of the job table
   std::vector<double*> diag;       // per job diagnostics buffer
   std::vector<int> nbB;
+  std::vector<long long> loops0;   // per job loop count at creation
 };
 
 hipEvent_t* prof_slots(BatchHost* b) {
@@ -818,7 +819,10 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
         bv = bvar[s];
         bl.ldsB[bv] = std::max(bl.ldsB[bv], zlds(bv));
       }
-      if (s == 0) b->nbB.push_back(bb);
+      if (s == 0) {
+        b->nbB.push_back(bb);
+        b->loops0.push_back(jobs[j].loops);
+      }
       pA[s][j + 1] = pA[s][j] + a;
       pC[s][j + 1] = pC[s][j] + c;
       pCn[s][j + 1] = pCn[s][j] + cn;
@@ -961,16 +965,28 @@ int mlv_batch_step(mlv_batch* bp, int nsteps, int write_phys, void* stream) {
   return MLV_OK;
 }
 
+// A job's CFL check loops: the loops whose count before them (the job's
+// loops at creation plus the batch's steps) is a multiple of cadence --
+// loops 1, 1+cadence, ... of Simulation.end_loop's ticker, counted from
+// the start of the simulation, so a resumed job keeps the phase its
+// uninterrupted run had.  Returns whether step `step` of the batch is a
+// check loop of any job.
+static bool any_check(const BatchHost* b, long long step, int cadence) {
+  if (cadence <= 0) return false;
+  for (long long l0 : b->loops0)
+    if ((l0 + step) % cadence == 0) return true;
+  return false;
+}
+
 // Steps first .. first+nsteps-1 (counted from the batch's creation).  With
-// cadence > 0 the steps whose count is a multiple of cadence (loops 1,
-// 1+cadence, ... of Simulation.end_loop's CFL ticker) also produce the
-// diagnostics partials and are followed by the on-device CFL update.
+// cadence > 0 the steps that are some job's check loop also produce that
+// job's diagnostics partials and are followed by the on-device CFL update.
 static hipError_t enqueue_steps(BatchHost* b, int nsteps, int write_phys, long long first,
                                 hipStream_t s, bool timed, int cadence, double cutoff) {
   for (int k = 0; k < nsteps; ++k) {
     const int wp = (write_phys && k == nsteps - 1) ? kWritePhys : 0;
-    const bool dg = cadence > 0 && ((first + k) % cadence) == 0;
-    const int flags0 = wp | (dg ? kDiag : 0);
+    const bool dg = any_check(b, first + k, cadence);
+    const int flags0 = wp | (dg ? (kDiag | (cadence << kDiagShift)) : 0);
     const int step = (int)((first + k) & 3);
     for (int st = 0; st < b->nstage; ++st) {
       hipEvent_t* ev = timed ? prof_slots(b) : nullptr;
@@ -984,8 +1000,10 @@ static hipError_t enqueue_steps(BatchHost* b, int nsteps, int write_phys, long l
     }
     if (dg) {
       hipError_t e = b->dtype == MLV_F64
-                         ? fused_diag<double>((JobDev<double>*)b->jobs_dev, b->njobs, cutoff, s)
-                         : fused_diag<float>((JobDev<float>*)b->jobs_dev, b->njobs, cutoff, s);
+                         ? fused_diag<double>((JobDev<double>*)b->jobs_dev, b->njobs, cutoff,
+                                              cadence, s)
+                         : fused_diag<float>((JobDev<float>*)b->jobs_dev, b->njobs, cutoff,
+                                             cadence, s);
       if (e != hipSuccess) return e;
     }
   }

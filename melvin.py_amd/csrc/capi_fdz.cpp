@@ -42,6 +42,7 @@ struct FdzHost {
   int max_nn = 0;
   std::vector<double*> diag;    // per job: partials, header, tracker records
   std::vector<int> nbC;         // per job: column blocks of phase 0
+  std::vector<long long> loops0;  // per job: loop count at creation
   bool prof = false;
   std::vector<hipEvent_t> events;
   size_t used = 0;
@@ -220,6 +221,7 @@ int create_impl(FdzHost* b, const mlv_fdz_job_desc* jobs, int njobs) {
   // diagnostics: partials of phase 0's column blocks, header, records
   b->diag.assign(njobs, nullptr);
   b->nbC.assign(njobs, 0);
+  b->loops0.assign(njobs, 0);
   for (int j = 0; j < njobs; ++j) {
     FdzJob<T>& J = host_job<T>(b, j);
     const size_t bytes = diag_doubles(J.nbC) * sizeof(double);
@@ -228,6 +230,7 @@ int create_impl(FdzHost* b, const mlv_fdz_job_desc* jobs, int njobs) {
     HIP_TRY(hipMemset(J.diag, 0, bytes), "clear diagnostics");
     b->diag[j] = J.diag;
     b->nbC[j] = J.nbC;
+    b->loops0[j] = J.loops;
   }
   // factor tables, chunk-major, one (cp, 1/den) pair per element
   for (int j = 0; j < njobs; ++j) {
@@ -263,10 +266,20 @@ int create_impl(FdzHost* b, const mlv_fdz_job_desc* jobs, int njobs) {
   return MLV_OK;
 }
 
+// A job's CFL check loops: the loops whose count before them (the job's
+// loops at creation plus the batch's steps) is a multiple of cadence --
+// loops 1, 1+cadence, ... of Simulation.end_loop's ticker, counted from
+// the start of the simulation, so a resumed job keeps its phase.
+bool any_check(const FdzHost* b, long long step, int cadence) {
+  if (cadence <= 0) return false;
+  for (long long l0 : b->loops0)
+    if ((l0 + step) % cadence == 0) return true;
+  return false;
+}
+
 // Steps first .. first+nsteps-1 (counted from the batch's creation).  With
-// cadence > 0 the steps whose count is a multiple of cadence (loops 1,
-// 1+cadence, ... of Simulation.end_loop's CFL ticker) also produce the
-// diagnostics partials and are followed by the on-device CFL update.
+// cadence > 0 the steps that are some job's check loop also produce that
+// job's diagnostics partials and are followed by the on-device CFL update.
 template <typename T>
 hipError_t enqueue(FdzHost* b, int nsteps, int write_phys, long long first, hipStream_t s,
                    bool timed, int cadence, double cutoff) {
@@ -274,8 +287,8 @@ hipError_t enqueue(FdzHost* b, int nsteps, int write_phys, long long first, hipS
   const cx<T>* tw = (const cx<T>*)b->tw;
   for (int k = 0; k < nsteps; ++k) {
     const int wp = (write_phys && k == nsteps - 1) ? kFdzWritePhys : 0;
-    const bool dg = cadence > 0 && ((first + k) % cadence) == 0;
-    const int flags0 = wp | (dg ? kFdzDiag : 0);
+    const bool dg = any_check(b, first + k, cadence);
+    const int flags0 = wp | (dg ? (kFdzDiag | (cadence << kFdzDiagShift)) : 0);
     const int step = (int)((first + k) & 3);
     for (int ph = 0; ph < b->nphase; ++ph) {
       hipEvent_t* ev = timed ? prof_slots(b) : nullptr;
@@ -283,7 +296,7 @@ hipError_t enqueue(FdzHost* b, int nsteps, int write_phys, long long first, hipS
       if (e != hipSuccess) return e;
     }
     if (dg) {
-      hipError_t e = fdz_diag<T>(jd, b->njobs, b->max_nn, cutoff, s);
+      hipError_t e = fdz_diag<T>(jd, b->njobs, b->max_nn, cutoff, cadence, s);
       if (e != hipSuccess) return e;
     }
   }

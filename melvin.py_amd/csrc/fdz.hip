@@ -474,7 +474,7 @@ __device__ __forceinline__ void fdz_cols_body(const FdzJob<T>& J, int local, int
     __syncthreads();
     fft_lds<T, LOGN, +1, kNT, P>(lds, nslot, tw);
     const bool wp = (flags & kFdzWritePhys) && !corr;
-    const bool dg = (flags & kFdzDiag) && !corr;
+    const bool dg = (flags & kFdzDiag) && !corr && J.loops % (flags >> kFdzDiagShift) == 0;
     T mux = -INFINITY, muz = -INFINITY;  // NaN-propagating maxima, as np.max
     double ke = 0.0, ens = 0.0, flux = 0.0;
     for (int e = threadIdx.x; e < npair * N; e += kNT) {
@@ -628,8 +628,13 @@ __global__ void k_fdz_factors(double2* __restrict__ fac, int nn, int nz, int K, 
 // column kernel's partials; a changed dt also moves the theta operators'
 // interior bands and flags the job for k_fdz_refactor.
 template <typename T>
-__global__ void __launch_bounds__(kNT) k_fdz_diag(FdzJob<T>* jobs, double cutoff) {
+__global__ void __launch_bounds__(kNT) k_fdz_diag(FdzJob<T>* jobs, double cutoff, int cadence) {
   FdzJob<T>& J = jobs[blockIdx.x];
+  // loops has been advanced by this step's last row kernel
+  if ((J.loops - 1) % cadence != 0) {
+    if (threadIdx.x == 0) J.refactor = 0;
+    return;
+  }
   double dt = J.dt;
   const bool changed = diag_apply<kNT>(J.diag, J.nbC, J.dx, J.dz, (double)J.nx * (double)J.nz,
                                        true, cutoff, J.t, J.loops, dt);
@@ -728,8 +733,9 @@ hipError_t fdz_set_dt(FdzJob<T>* jobs, int job, double dt, double lo_w, double l
 }
 
 template <typename T>
-hipError_t fdz_diag(FdzJob<T>* jobs, int njobs, int max_nn, double cutoff, hipStream_t s) {
-  hipLaunchKernelGGL(k_fdz_diag<T>, dim3(njobs), dim3(kNT), 0, s, jobs, cutoff);
+hipError_t fdz_diag(FdzJob<T>* jobs, int njobs, int max_nn, double cutoff, int cadence,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_fdz_diag<T>, dim3(njobs), dim3(kNT), 0, s, jobs, cutoff, cadence);
   hipLaunchKernelGGL(k_fdz_refactor<T>, dim3((max_nn + 63) / 64, njobs, 2), dim3(64), 0, s,
                      (const FdzJob<T>*)jobs);
   return hipGetLastError();
@@ -757,8 +763,8 @@ template hipError_t fdz_prologue<float>(const FdzJob<float>*, const FdzLaunch&, 
 template hipError_t fdz_prologue<double>(const FdzJob<double>*, const FdzLaunch&, int, hipStream_t);
 template hipError_t fdz_set_dt<float>(FdzJob<float>*, int, double, double, double, hipStream_t);
 template hipError_t fdz_set_dt<double>(FdzJob<double>*, int, double, double, double, hipStream_t);
-template hipError_t fdz_diag<float>(FdzJob<float>*, int, int, double, hipStream_t);
-template hipError_t fdz_diag<double>(FdzJob<double>*, int, int, double, hipStream_t);
+template hipError_t fdz_diag<float>(FdzJob<float>*, int, int, double, int, hipStream_t);
+template hipError_t fdz_diag<double>(FdzJob<double>*, int, int, double, int, hipStream_t);
 template hipError_t fdz_prepare<float>(const FdzLaunch&);
 template hipError_t fdz_prepare<double>(const FdzLaunch&);
 

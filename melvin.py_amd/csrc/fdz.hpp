@@ -70,7 +70,9 @@ struct FdzJob {
 };
 
 // flags of phase 0's column kernel: physical outputs, diagnostics partials
-constexpr int kFdzWritePhys = 1, kFdzDiag = 2;
+// (with the CFL cadence from bit kFdzDiagShift up: a job writes partials on
+// the loops whose count before the loop is a multiple of it)
+constexpr int kFdzWritePhys = 1, kFdzDiag = 2, kFdzDiagShift = 8;
 
 struct FdzLaunch {
   int njobs;
@@ -93,11 +95,13 @@ inline int fdz_cols_per_block(int lognx) {
 template <typename T>
 hipError_t fdz_phase(const FdzJob<T>* jobs, const FdzLaunch& fl, int phase, int flags, int step,
                      const cx<T>* tw, hipStream_t s, hipEvent_t* ev = nullptr);
-// After a step whose phase-0 column kernel ran with kFdzDiag: each job's
-// tracker record and CFL rule (dt, theta bands), then the theta tables of
-// the jobs whose dt changed (max_nn = largest nn of the batch).
+// After a step whose phase-0 column kernel ran with kFdzDiag: for each job
+// whose check loop it was, the tracker record and CFL rule (dt, theta
+// bands), then the theta tables of the jobs whose dt changed (max_nn =
+// largest nn of the batch).
 template <typename T>
-hipError_t fdz_diag(FdzJob<T>* jobs, int njobs, int max_nn, double cutoff, hipStream_t s);
+hipError_t fdz_diag(FdzJob<T>* jobs, int njobs, int max_nn, double cutoff, int cadence,
+                    hipStream_t s);
 template <typename T>
 hipError_t fdz_prologue(const FdzJob<T>* jobs, const FdzLaunch& fl, int step, hipStream_t s);
 // (Re)build the factor tables of one job: which = 0 Poisson, 1 theta(omega),

@@ -188,7 +188,8 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     }
     __syncthreads();
     fft_lds<T, LOGN, +1, kNT, P>(lds, nslot, tw);
-    const bool diag = flags & kDiag;
+    // this job's check loop (J.loops counts the loops before this one)
+    const bool diag = (flags & kDiag) && J.loops % (flags >> kDiagShift) == 0;
     T mux = -INFINITY, muz = -INFINITY;  // NaN-propagating maxima, as np.max
     double ke = 0.0, ens = 0.0, flux = 0.0;
     for (int n = threadIdx.x; n < N; n += kNT) {
@@ -599,8 +600,11 @@ __global__ void k_set_dt(JobDev<T>* jobs, int j, double dt) {
 // tracker record and CFL rule (diag_apply).  The first advected scalar
 // exists in vorticity and DDC jobs of two or more fields.
 template <typename T>
-__global__ void __launch_bounds__(kNT) k_diag(JobDev<T>* jobs, double cutoff) {
+__global__ void __launch_bounds__(kNT) k_diag(JobDev<T>* jobs, double cutoff, int cadence) {
   JobDev<T>& J = jobs[blockIdx.x];
+  // loops has been advanced by this step's update kernel: the step was this
+  // job's check loop when the count before it was a multiple of cadence
+  if ((J.loops - 1) % cadence != 0) return;
   double dt = J.dt;
   const bool flux = J.problem != MLV_PROB_MHD && J.nfield >= 2;
   if (diag_apply<kNT>(J.diag, J.nbB, J.dx, J.dz, (double)J.nx * (double)J.nz, flux, cutoff, J.t,
@@ -678,12 +682,12 @@ hipError_t fused_set_dt(JobDev<T>* jobs, int job, double dt, hipStream_t s) {
 }
 
 template <typename T>
-hipError_t fused_diag(JobDev<T>* jobs, int njobs, double cutoff, hipStream_t s) {
-  hipLaunchKernelGGL(k_diag<T>, dim3(njobs), dim3(kNT), 0, s, jobs, cutoff);
+hipError_t fused_diag(JobDev<T>* jobs, int njobs, double cutoff, int cadence, hipStream_t s) {
+  hipLaunchKernelGGL(k_diag<T>, dim3(njobs), dim3(kNT), 0, s, jobs, cutoff, cadence);
   return hipGetLastError();
 }
-template hipError_t fused_diag<float>(JobDev<float>*, int, double, hipStream_t);
-template hipError_t fused_diag<double>(JobDev<double>*, int, double, hipStream_t);
+template hipError_t fused_diag<float>(JobDev<float>*, int, double, int, hipStream_t);
+template hipError_t fused_diag<double>(JobDev<double>*, int, double, int, hipStream_t);
 template hipError_t fused_set_dt<float>(JobDev<float>*, int, double, hipStream_t);
 template hipError_t fused_set_dt<double>(JobDev<double>*, int, double, hipStream_t);
 template hipError_t fused_stage<float>(const JobDev<float>*, const BatchLaunch&, int, int, int,

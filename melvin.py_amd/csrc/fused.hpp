@@ -222,12 +222,16 @@ hipError_t fused_stage(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, 
                        int step, const cx<T>* tw, hipStream_t s, hipEvent_t* ev = nullptr);
 template <typename T>
 hipError_t fused_set_dt(JobDev<T>* jobs, int job, double dt, hipStream_t s);
-// flags of a stage's z pass: physical outputs, diagnostics partials
-constexpr int kWritePhys = 1, kDiag = 2;
-// After a step whose z pass ran with kDiag: reduce each job's partials,
-// append a tracker record, and apply the CFL rule to the job's dt.
+// flags of a stage's z pass: physical outputs, diagnostics partials; with
+// kDiag the CFL cadence sits in the bits from kDiagShift up, and a job
+// writes partials on the loops whose count before the loop is a multiple of
+// it (the reference's ticker fires after loops 1, 1 + cadence, ...)
+constexpr int kWritePhys = 1, kDiag = 2, kDiagShift = 8;
+// After a step whose z pass ran with kDiag: for each job whose check loop
+// it was, reduce its partials, append a tracker record, and apply the CFL
+// rule to its dt.
 template <typename T>
-hipError_t fused_diag(JobDev<T>* jobs, int njobs, double cutoff, hipStream_t s);
+hipError_t fused_diag(JobDev<T>* jobs, int njobs, double cutoff, int cadence, hipStream_t s);
 template <typename T>
 hipError_t fused_prepare(const BatchLaunch& bl);
 

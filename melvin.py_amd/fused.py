@@ -383,27 +383,32 @@ def run_device(batch, nsteps, cfl_cadence=10, cutoff=0.5, chunk=None):
 
 def run(batch, nsteps, cfl_cadence=10, cutoff=0.5, on_cfl=None, cfl=None):
     """Advance every job ``nsteps`` loops with the reference's CFL ticker
-    (Simulation.py:99-109, Ticker.py:14-25): dt is checked after loops 1,
-    1+cadence, ... from that loop's velocities and applies from the next
-    loop.  Jobs need ``want_phys``; cfl_cadence=0 disables the check.
+    (Simulation.py:99-109, Ticker.py:14-25): a job's dt is checked after its
+    loops 1, 1+cadence, ... from that loop's velocities and applies from the
+    next loop.  Loops count from the start of the job's simulation
+    (``job.steps``), so a job resumed from a checkpoint keeps the phase of
+    its uninterrupted run, as the reference's restored ticker does.  Jobs
+    need ``want_phys``; cfl_cadence=0 disables the check.
     ``cfl(job, dt, cutoff)`` is the dt rule (default: cfl_dt)."""
     cfl_dt_fn = cfl if cfl is not None else cfl_dt
     done = 0
-    next_check = 1
     while done < nsteps:
-        if cfl_cadence and done + 1 == next_check:
-            batch.step(1, write_phys=True)
-            done += 1
-            for i, job in enumerate(batch.jobs):
-                new = cfl_dt_fn(job, job.dt, cutoff)
-                if new != job.dt:
-                    batch.set_dt(i, new)
-                if on_cfl is not None:
-                    on_cfl(i, job)
-            next_check += cfl_cadence
+        # loops before the next check loop of any job (a job's check loop
+        # starts with its loop count a multiple of the cadence)
+        k = min((-j.steps) % cfl_cadence for j in batch.jobs) if cfl_cadence else nsteps
+        k = min(k, nsteps - done)
+        if k > 0:
+            batch.step(k)
+            done += k
             continue
-        k = (next_check - 1 - done) if cfl_cadence else nsteps - done
-        k = min(max(k, 1), nsteps - done)
-        batch.step(k)
-        done += k
+        batch.step(1, write_phys=True)
+        done += 1
+        for i, job in enumerate(batch.jobs):
+            if (job.steps - 1) % cfl_cadence:
+                continue
+            new = cfl_dt_fn(job, job.dt, cutoff)
+            if new != job.dt:
+                batch.set_dt(i, new)
+            if on_cfl is not None:
+                on_cfl(i, job)
     return done

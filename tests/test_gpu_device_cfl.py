@@ -239,3 +239,63 @@ def test_fdz_device_ticker_reports_a_breach(mlv):
     with pytest.raises(Exception, match="CFL condition breached"):
         mlv.fdz.run_device(b, 20, chunk=20)
     b.close()
+
+
+def _pc_case(mlv, dtype="f64"):
+    p = O.make_params(dict(nx=64, nz=65, lx=2.0, lz=1.0, Pr=1.0, Ra=1e4, alpha=0.51,
+                           integrator_order=2, discretisation=["spectral", "fdm"]))
+    X, Z = O.grid(p)
+    t0 = O.to_spectral(p, 1 - Z + O.noise((p.nx, p.nz), 1e-2, 5))
+    w0 = O.to_spectral(p, 400.0 * np.sin(np.pi * X) * np.sin(np.pi * Z))
+    j = _fdz_job(mlv, p, "pc", dtype=dtype, dt=2e-4, want_phys=True)
+    j.set_state([w0, t0])
+    return j
+
+
+@pytest.mark.parametrize("kind", ["fused", "fdz"])
+@pytest.mark.parametrize("ticker", ["host", "device"])
+def test_resumed_run_keeps_the_ticker_phase(mlv, tmp_path, kind, ticker):
+    """A run checkpointed after 15 loops and resumed for 18 more equals the
+    uninterrupted 33-loop run, CFL ticker included (checks stay at loops 1,
+    11, 21, 31 as the reference's restored Ticker keeps them); the resumed
+    job batched with a fresh one equals both solo runs."""
+    if kind == "fused":
+        make = lambda: _tgv(mlv)[2]  # noqa: E731
+        Batch, mod = mlv.fused.Batch, mlv.fused
+    else:
+        make = lambda: _pc_case(mlv)  # noqa: E731
+        Batch, mod = mlv.fdz.FdzBatch, mlv.fdz
+    go = (lambda b, n: mod.run(b, n)) if ticker == "host" else (lambda b, n: mod.run_device(b, n))
+    ref = make()
+    b = Batch([ref])
+    go(b, 33)
+    b.close()
+    part = make()
+    b = Batch([part])
+    go(b, 15)
+    f = str(tmp_path / "ck.npz")
+    mlv.checkpoint.save(part, f)
+    b.close()
+    print(f"[device-cfl] resume {kind} {ticker}", flush=True)
+    res = mlv.checkpoint.load(f, want_phys=True)
+    assert res.steps == 15
+    b = Batch([res])
+    go(b, 18)
+    b.close()
+    assert res.dt == ref.dt
+    for x, y in zip(res.state(), ref.state()):
+        np.testing.assert_array_equal(x, y)
+    # a resumed job and a fresh one in one batch: each as alone
+    res2 = mlv.checkpoint.load(f, want_phys=True)
+    fresh, solo = make(), make()
+    b = Batch([solo])
+    go(b, 18)
+    b.close()
+    b = Batch([fresh, res2])
+    go(b, 18)
+    b.close()
+    for x, y in zip(res2.state(), ref.state()):
+        np.testing.assert_array_equal(x, y)
+    for x, y in zip(fresh.state(), solo.state()):
+        np.testing.assert_array_equal(x, y)
+    assert fresh.dt == solo.dt and res2.dt == ref.dt
