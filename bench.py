@@ -10,7 +10,8 @@ independent simulations (distinct noise seeds) in FP32.  A "step" advances
 every simulation of the batch by one timestep; `value` counts grid points x
 steps over the whole batch.  The same workload (one simulation) in FP64 is
 timed after it as the plain path, and the FP32 result of simulation 0 is
-compared with the FP64 one after the timed steps.
+compared with the FP64 one after the same steps (warm-up, timed, and the
+kernel-timing steps that follow the timed region).
 
 Printed: one JSON line with the metric, the dominant kernel's measured
 bandwidth against the HBM roofline, and a CPU baseline (the NumPy FP64
@@ -332,15 +333,20 @@ def main():
         return jobs
 
     def timed(batch, steps, warmup):
+        """Wall time of ``steps`` steps (no per-kernel events inside the timed
+        region), then the average duration of each kernel over a separate
+        run of up to 50 steps timed with HIP events on the launch stream."""
+        import ctypes
+
         batch.step(warmup)
         torch.cuda.synchronize()
-        m._lib.call("mlv_batch_profile", batch.handle, 1)
         t0 = time.perf_counter()
         batch.step(steps)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
-        import ctypes
-
+        m._lib.call("mlv_batch_profile", batch.handle, 1)
+        batch.step(min(steps, 50))
+        torch.cuda.synchronize()
         ms = (ctypes.c_double * 3)()
         n = (ctypes.c_longlong * 3)()
         m._lib.call("mlv_batch_profile_read", batch.handle, ms, n)
