@@ -184,7 +184,7 @@ int create_impl(FdzHost* b, const mlv_fdz_job_desc* jobs, int njobs) {
     int rc = build_fdz_job<T>(jobs[j], J, j);
     if (rc) return rc;
     const int tc = fdz_cols_per_block<T>(J.lognx);
-    const size_t ldsc = fft_lds_bytes<T>(J.lognx);
+    const size_t ldsc = fdz_cols_lds_bytes<T>(J.lognx);
     if (tc < 1 || ldsc > lds_max)
       return fail(MLV_ERR_UNSUPPORTED, "job " + std::to_string(j) + ": nx too large for LDS");
     fl.ldscol = std::max(fl.ldscol, ldsc);

@@ -408,7 +408,7 @@ __device__ __forceinline__ void fdz_cols_body(const FdzJob<T>& J, int local, int
                                               int step, const cx<T>* __restrict__ tw,
                                               cx<T>* lds) {
   constexpr int N = 1 << LOGN;
-  constexpr int P = fft_points_col<T>(LOGN);
+  constexpr int P = fdz_points<T>(LOGN);
   constexpr int SL = fft_slots<LOGN, kNT, P>();
   constexpr int TC = fdz_tc<T>(SL);
   if constexpr (TC < 2) {

@@ -84,12 +84,23 @@ struct FdzLaunch {
   int *prefrow[3][kFdzVariants];
 };
 
+// Points per thread of the column kernel: 16 below N = 2048 (two columns
+// per block at 1024, so a single job's z columns spread over twice as many
+// CUs as at 32 points), 32 at 2048 (a block must hold a column pair).
+template <typename T>
+__host__ __device__ constexpr int fdz_points(int lognx) {
+  return lognx >= 11 ? 32 : 16;
+}
 // Columns per block of the column kernel (two LDS slots each, in pairs of
 // columns); 0 when the LDS slots do not hold two columns.
 template <typename T>
 inline int fdz_cols_per_block(int lognx) {
-  const int s = slots_for<T>(lognx) / 2;
+  const int s = ((kNT * fdz_points<T>(lognx)) >> lognx) / 2;
   return s < 2 ? 0 : (s > kNT ? kNT : s);
+}
+template <typename T>
+inline size_t fdz_cols_lds_bytes(int lognx) {
+  return (size_t)((kNT * fdz_points<T>(lognx)) >> lognx) * fft_ldsn(1 << lognx) * sizeof(cx<T>);
 }
 
 template <typename T>
