@@ -435,7 +435,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   // spectral state is kz-major, so a column is contiguous).  Loads of U2
   // row groups are issued before any is used.
   constexpr int KR = (2 * ((N - 1) / 3) + 1 + 63) / 64;  // bound on ceil(rows / 64)
-  constexpr int U2 = WIDE ? 4 : 2;
+  constexpr int U2 = 4;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const bool ddc = J.problem == MLV_PROB_DDC;
   uint32_t* __restrict__ carry = sizeof(T) == 4 ? J.comp[field] : nullptr;
