@@ -333,12 +333,17 @@ def main():
         return jobs
 
     def timed(batch, steps, warmup):
-        """Wall time of ``steps`` steps (no per-kernel events inside the timed
-        region), then the average duration of each kernel over a separate
-        run of up to 50 steps timed with HIP events on the launch stream."""
+        """The warm-up picks the update kernel's tile shape by measurement
+        (Batch.autotune: real steps with each shape) and steps on to
+        ``warmup`` steps; then the wall time of ``steps`` steps (no
+        per-kernel events inside the timed region), then the average
+        duration of each kernel over a separate run of up to 50 steps timed
+        with HIP events on the launch stream."""
         import ctypes
 
-        batch.step(warmup)
+        tune = max(1, warmup // 4)
+        shapes.append(batch.autotune(steps=tune))
+        batch.step(max(0, warmup - 2 * tune))
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         batch.step(steps)
@@ -352,6 +357,8 @@ def main():
         m._lib.call("mlv_batch_profile_read", batch.handle, ms, n)
         m._lib.call("mlv_batch_profile", batch.handle, 0)
         return el, [ms[k] / max(n[k], 1) for k in range(3)], [n[k] for k in range(3)]
+
+    shapes = []  # update kernel tile shape picked for each leg
 
     # batched FP32 (the headline)
     jobs32 = make_jobs(args.batch, "f32")
@@ -407,6 +414,7 @@ def main():
         "ratio_batched_fp32_vs_plain_fp64": value / value64,
         "rel_err_fp32_vs_fp64_omega": err,
         "kernel_ms_per_launch": dict(zip(names, kms32)),
+        "update_kernel_shape": {"fp32_batched": shapes[0], "fp64_plain": shapes[1]},
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(names[dom], args.batch),

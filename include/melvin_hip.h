@@ -213,6 +213,13 @@ int mlv_batch_step(mlv_batch* batch, int nsteps, int write_phys, void* stream);
 int mlv_batch_graph_step(mlv_batch* batch, int nsteps, void* stream);
 /* Set one job's dt for the following steps (stream-ordered). */
 int mlv_batch_set_dt(mlv_batch* batch, int job, double dt, void* stream);
+/* Pick the update kernel's tile shape for this batch by measurement: runs
+ * nsteps real steps with each shape (both compute bit-identical results, so
+ * the steps advance the simulations as mlv_batch_step would), keeps the
+ * faster and remembers it for batches created later for the same set of job
+ * shapes (process-wide).  *chosen = 1 for the wide shape (32 points per
+ * thread where it applies), 0 for the narrow one.  Synchronises. */
+int mlv_batch_autotune(mlv_batch* batch, int nsteps, void* stream, int* chosen);
 /* nsteps loops with Simulation.end_loop's CFL ticker on the device
  * (Simulation.py:199-209, Ticker.py:14-25, Integrator.py:35-44): after each
  * job's loops 1, 1+cadence, ... -- counted from the job descriptor's loops,

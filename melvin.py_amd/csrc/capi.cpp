@@ -2,6 +2,7 @@
 // include/melvin_hip.h): plans, operator-level calls, fused job batches.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -551,7 +552,29 @@ struct BatchHost {
   std::vector<double*> diag;       // per job diagnostics buffer
   std::vector<int> nbB;
   std::vector<long long> loops0;   // per job loop count at creation
+  std::string signature;           // the job set's shapes (batch_signature)
 };
+
+// Kernel C tile shapes picked by mlv_batch_autotune, by job-set signature;
+// a batch created later for the same set of shapes starts with the choice.
+std::mutex g_tuned_mu;
+std::map<std::string, int> g_tuned;
+
+// Precision and the sorted (problem, fields, nx, nz, order) of every job:
+// what the kernels' block grids depend on.
+std::string batch_signature(int dtype, const mlv_job_desc* jobs, int njobs) {
+  std::vector<std::string> parts;
+  for (int j = 0; j < njobs; ++j) {
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%d:%d:%d:%d:%d", jobs[j].problem, jobs[j].nfield, jobs[j].nx,
+                  jobs[j].nz, jobs[j].ab_order);
+    parts.push_back(buf);
+  }
+  std::sort(parts.begin(), parts.end());
+  std::string s = std::to_string(dtype);
+  for (const auto& p : parts) s += "|" + p;
+  return s;
+}
 
 hipEvent_t* prof_slots(BatchHost* b) {
   if (!b->prof) return nullptr;
@@ -846,10 +869,15 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
         cus <= 0)
       cus = 256;
     bl.wideC = (b->dtype == MLV_F64 && pC[0][njobs] >= 4 * cus) ? 1 : 0;
-    if (!bl.wideC) {
-      for (int s = 0; s < 2; ++s) pC[s] = pCn[s];
-      bl.ldsC = ldsCn;
+    // a shape picked by mlv_batch_autotune for the same set of jobs wins
+    b->signature = batch_signature(dtype, jobs, njobs);
+    {
+      std::lock_guard<std::mutex> lk(g_tuned_mu);
+      auto it = g_tuned.find(b->signature);
+      if (it != g_tuned.end()) bl.wideC = it->second;
     }
+    bl.ldsCp[1] = bl.ldsC;
+    bl.ldsCp[0] = ldsCn;
   }
   if (rc) {
     delete b;
@@ -870,14 +898,17 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
   rc = dalloc(host.size(), &b->jobs_dev);
   for (int s = 0; s < 2 && !rc; ++s) {
     bl.nA[s] = pA[s][njobs];
-    bl.nC[s] = pC[s][njobs];
+    bl.nCp[1][s] = pC[s][njobs];
+    bl.nCp[0][s] = pCn[s][njobs];
     rc = upload_pref(pA[s], &bl.prefA[s]);
-    if (!rc) rc = upload_pref(pC[s], &bl.prefC[s]);
+    if (!rc) rc = upload_pref(pC[s], &bl.prefCp[1][s]);
+    if (!rc) rc = upload_pref(pCn[s], &bl.prefCp[0][s]);
     for (int v = 0; v < BV_COUNT && !rc; ++v) {
       bl.nB[s][v] = pB[s][v][njobs];
       rc = upload_pref(pB[s][v], &bl.prefB[s][v]);
     }
   }
+  if (!rc) select_c_policy(bl, bl.wideC);
   for (int j = 0; j < njobs && !rc; ++j) {
     double* dg = nullptr;
     const size_t bytes = diag_doubles(b->nbB[j]) * sizeof(double);
@@ -1045,6 +1076,43 @@ static int graph_run(BatchHost* b, int nsteps, int cadence, double cutoff, void*
 int mlv_batch_graph_step(mlv_batch* bp, int nsteps, void* stream) {
   if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
   return graph_run(reinterpret_cast<BatchHost*>(bp), nsteps, 0, 0.0, stream);
+}
+
+int mlv_batch_autotune(mlv_batch* bp, int nsteps, void* stream, int* chosen) {
+  if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
+  if (nsteps < 1) return fail(MLV_ERR_ARG, "nsteps must be positive");
+  BatchHost* b = reinterpret_cast<BatchHost*>(bp);
+  hipStream_t s = (hipStream_t)stream;
+  hipEvent_t ev[2];
+  HIP_TRY(hipEventCreate(&ev[0]), "hipEventCreate");
+  HIP_TRY(hipEventCreate(&ev[1]), "hipEventCreate");
+  float ms[2] = {0.f, 0.f};
+  hipError_t e = hipSuccess;
+  for (int wide = 0; wide < 2 && e == hipSuccess; ++wide) {
+    select_c_policy(b->bl, wide);
+    e = hipEventRecord(ev[0], s);
+    if (e == hipSuccess) e = enqueue_steps(b, nsteps, 0, b->steps, s, false);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], s);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms[wide], ev[0], ev[1]);
+    if (e == hipSuccess) b->steps += nsteps;
+  }
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
+  if (e != hipSuccess) return hip_fail(e, "autotune");
+  const int best = ms[1] < ms[0] ? 1 : 0;
+  select_c_policy(b->bl, best);
+  // a captured graph holds the launches of the shape it was captured with
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_tuned_mu);
+    g_tuned[b->signature] = best;
+  }
+  if (chosen) *chosen = best;
+  return MLV_OK;
 }
 
 int mlv_batch_run(mlv_batch* bp, int nsteps, int cadence, double cutoff, int graph, void* stream) {

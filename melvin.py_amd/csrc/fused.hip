@@ -662,8 +662,8 @@ hipError_t fused_prepare(const BatchLaunch& bl) {
   size_t b = 0;
   for (int v = 0; v < BV_COUNT; ++v) b = bl.ldsB[v] > b ? bl.ldsB[v] : b;
   if (bl.ldsA) MLV_SET_LDS_ONCE(k_stepA<T>, bl.ldsA);
-  if (bl.ldsC && bl.wideC) MLV_SET_LDS_ONCE((k_stepC<T, 1>), bl.ldsC);
-  if (bl.ldsC && !bl.wideC) MLV_SET_LDS_ONCE((k_stepC<T, 0>), bl.ldsC);
+  if (bl.ldsCp[1]) MLV_SET_LDS_ONCE((k_stepC<T, 1>), bl.ldsCp[1]);
+  if (bl.ldsCp[0]) MLV_SET_LDS_ONCE((k_stepC<T, 0>), bl.ldsCp[0]);
   if (bl.ldsB[BV_VORT1]) MLV_SET_LDS_ONCE((k_stepB<T, BV_VORT1>), bl.ldsB[BV_VORT1]);
   if (bl.ldsB[BV_VORT2]) MLV_SET_LDS_ONCE((k_stepB<T, BV_VORT2>), bl.ldsB[BV_VORT2]);
   if (bl.ldsB[BV_VORT3]) MLV_SET_LDS_ONCE((k_stepB<T, BV_VORT3>), bl.ldsB[BV_VORT3]);

@@ -212,7 +212,22 @@ struct BatchLaunch {
   int* prefA[2];                       // device prefix arrays (njobs + 1)
   int* prefB[2][BV_COUNT];
   int* prefC[2];
+  // both tile shapes of kernel C (index = wide); wideC / nC / ldsC / prefC
+  // above are the selected one
+  int nCp[2][2];
+  size_t ldsCp[2];
+  int* prefCp[2][2];
 };
+
+// Select kernel C's tile shape (wide = 1: fused_points' wide policy).
+inline void select_c_policy(BatchLaunch& bl, int wide) {
+  bl.wideC = wide;
+  bl.ldsC = bl.ldsCp[wide];
+  for (int s = 0; s < 2; ++s) {
+    bl.nC[s] = bl.nCp[wide][s];
+    bl.prefC[s] = bl.prefCp[wide][s];
+  }
+}
 
 // `step` is the batch's step count modulo 4 (a multiple of both ring
 // lengths and of the ping-pong period), so a captured sequence of 4k steps

@@ -320,6 +320,21 @@ class Batch(DeviceRun):
         _lib.call("mlv_batch_set_dt", self.handle, int(index), float(dt),
                   _lib.stream_handle(stream))
 
+    def autotune(self, steps=4, stream=None):
+        """Pick the update kernel's tile shape by timing ``steps`` real steps
+        with each (mlv_batch_autotune); the jobs advance by 2 * steps.
+        Returns "wide" or "narrow"."""
+        chosen = ctypes.c_int()
+        _lib.call("mlv_batch_autotune", self.handle, int(steps), _lib.stream_handle(stream),
+                  ctypes.byref(chosen))
+        n = 2 * int(steps)
+        for j in self.jobs:
+            j.steps += n
+            j.curr = (j.curr + n) % j.ab_order
+            j.parity = (j.parity + n) & 1
+            j.t += n * j.dt
+        return "wide" if chosen.value else "narrow"
+
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:
             _lib.lib().mlv_batch_destroy(self.handle)

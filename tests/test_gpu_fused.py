@@ -232,3 +232,26 @@ def test_every_program_at_2048_vs_oracle(mlv, problem, nx, nz):
     for x, y in zip(mixed.state(), job.state()):
         np.testing.assert_array_equal(x, y)
     b.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_autotune_steps_are_real_steps(mlv, dtype):
+    """mlv_batch_autotune times real steps with both tile shapes of the
+    update kernel: afterwards every job equals the same number of plain
+    steps bit for bit (the shapes compute identical values), and a batch of
+    the same job shapes created later starts with the chosen shape."""
+    tuned = [j for j, _, _ in _golden_jobs(mlv, dtype)]
+    plain = [j for j, _, _ in _golden_jobs(mlv, dtype)]
+    b = mlv.fused.Batch(tuned)
+    print(f"[fused] autotune {dtype}", flush=True)
+    choice = b.autotune(steps=3)
+    assert choice in ("wide", "narrow")
+    b.step(4)
+    b.close()
+    b = mlv.fused.Batch(plain)
+    b.step(10)
+    b.close()
+    for t, p in zip(tuned, plain):
+        assert t.steps == p.steps == 10
+        for x, y in zip(t.state(), p.state()):
+            np.testing.assert_array_equal(x, y)
