@@ -271,6 +271,34 @@ def serving_leg(m, args):
     stats = srv.run(reqs, realtime=True)
     stats.update(arrival_rate_per_s=20.0, arrival_window_s=args.serving_seconds,
                  offered_grid_pt_steps=sum(r.work for r in reqs))
+    # FP32 against FP64 for the shortest vortex and MHD requests of the
+    # stream, run alone (a job's result is the same alone and in the mix)
+    # (every field; and, for scale, the FP64 run started from the FP32-rounded
+    # initial state)
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    errs = {}
+    for prob in ("vortex", "mhd"):
+        cand = [r for r in reqs if r.problem == prob]
+        if not cand:
+            continue
+        r = min(cand, key=lambda q: q.work)
+        out = {}
+        for dtype in ("f32", "f64", "f64_rounded_start"):
+            job = S.make_job(r, dtype[:3])
+            if dtype == "f64_rounded_start":
+                job.set_state([f.astype(np.complex64).astype(np.complex128) for f in job.state()])
+            b = m.fused.Batch([job])
+            b.step(r.steps)
+            out[dtype] = job.state()
+            b.close()
+        names = ("omega",) if prob == "vortex" else ("omega", "j")
+        errs[f"{prob}_{r.n}x{r.n}_{r.steps}_steps"] = {
+            "fp32_vs_fp64": {k: rel(out["f32"][i], out["f64"][i]) for i, k in enumerate(names)},
+            "fp64_rounded_start_vs_fp64": {
+                k: rel(out["f64_rounded_start"][i], out["f64"][i]) for i, k in enumerate(names)}}
+    stats["rel_err_fp32_vs_fp64"] = errs
     print(f"[bench] serving: {stats}", file=sys.stderr, flush=True)
     return stats
 
