@@ -198,6 +198,11 @@ typedef struct mlv_job_desc {
   int parity;          /* state buffer holding the current state */
   double t;            /* simulated time so far (advanced on the device) */
   long long loops;     /* loops so far (advanced on the device) */
+  int mean_split;      /* F32 only: carry each z-pass channel's x mean (kx = 0
+                          row) apart from its fluctuation, so that products
+                          with an O(1) x-independent profile keep FP32
+                          accuracy relative to the fluctuation; costs a few
+                          per cent of the step.  Ignored for F64. */
 } mlv_job_desc;
 
 int mlv_job_work_bytes(int problem, int nfield, int nx, int nz, int dtype, size_t* bytes);

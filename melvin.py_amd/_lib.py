@@ -77,6 +77,7 @@ class JobDesc(ctypes.Structure):
         ("parity", ctypes.c_int),
         ("t", ctypes.c_double),
         ("loops", ctypes.c_longlong),
+        ("mean_split", ctypes.c_int),
     ]
 
 

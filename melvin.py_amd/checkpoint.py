@@ -47,7 +47,7 @@ def _meta(job):
          "dt": job.dt, "t": job.t, "steps": job.steps, "curr": job.curr}
     if isinstance(job, fused.Job):
         m.update(kind="fused", problem=job.problem, nfield=job.nfield, coeff=list(job.coeff),
-                 param=list(job.param))
+                 param=list(job.param), mean_split=job.mean_split)
     else:
         m.update(kind="fdz", Pr=job.Pr, Ra=job.Ra, scheme=job.scheme)
     return m
@@ -119,7 +119,8 @@ def load(fname, **kw):
         job = fused.Job(m["problem"], m["nx"], m["nz"], m["lx"], m["lz"], dtype=m["dtype"],
                         dt=m["dt"], coeff=m["coeff"], param=m["param"], nfield=m["nfield"],
                         ab_order=m["ab_order"], fd_order=m["fd_order"], alpha=m["alpha"],
-                        compensated="carry0" in d.files, **kw)
+                        compensated="carry0" in d.files, mean_split=m.get("mean_split", True),
+                        **kw)
         nf = job.nfield
     else:
         job = fdz.FdzJob(m["nx"], m["nz"], m["lx"], m["lz"], m["Pr"], m["Ra"], dtype=m["dtype"],

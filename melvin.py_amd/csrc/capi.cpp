@@ -682,6 +682,7 @@ int build_job(const mlv_job_desc& d, JobDev<T>& J, int j) {
   J.psi = (cx<T>*)d.psi;
   J.t = d.t;
   J.loops = d.loops;
+  J.split = (sizeof(T) == 4 && d.mean_split) ? 1 : 0;
   for (int i = 0; i < kMaxPhys; ++i) J.phys[i] = (T*)d.phys[i];
   for (int s = 0; s < 2; ++s)
     for (int i = 0; i < kMaxPhys; ++i) J.st[s].phys_ch[i] = -1;
@@ -720,6 +721,10 @@ int build_job(const mlv_job_desc& d, JobDev<T>& J, int j) {
       if (J.phys[i]) s0.phys_ch[i] = i;
   }
   J.pbase = bvar_nin(J.st[0].bvar);
+  {
+    const int nin = bvar_nin(J.st[0].bvar), neq = bvar_neq(J.st[0].bvar);
+    J.mean = (T*)(J.work + (fused_work_elems(nin, neq, J.nx, J.nz) - (size_t)(kMeanCh / 2) * J.nz));
+  }
   return MLV_OK;
 }
 
@@ -737,8 +742,7 @@ int mlv_job_work_bytes(int problem, int nfield, int nx, int nz, int dtype, size_
     nf = nfield;
   }
   const int bv = problem == MLV_PROB_MHD ? BV_MHD1 : BV_VORT1 + (nf - 1);
-  const int nm = (nz - 1) / 3;
-  *bytes = (size_t)(bvar_nin(bv) + 2 * bvar_neq(bv)) * nx * nm * cplx_bytes(dtype);
+  *bytes = fused_work_elems(bvar_nin(bv), bvar_neq(bv), nx, nz) * cplx_bytes(dtype);
   return MLV_OK;
 }
 
@@ -853,6 +857,9 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
     }
     if (rc) break;
     bl.ldsA = std::max(bl.ldsA, ldsn);
+    // kernel M (single precision mean profiles): one LDS slot of nz
+    if (!f64 && jobs[j].mean_split)
+      bl.ldsM = std::max(bl.ldsM, (size_t)fft_ldsn(1 << lognz) * 8);
     bl.ldsC = std::max(bl.ldsC, ldsw);
     ldsCn = std::max(ldsCn, ldsn);
   }

@@ -82,6 +82,10 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
             v = mk<T>(-(yx * ps.y), yx * ps.x);
           }
         }
+        // single precision: the channel's x mean (its kx = 0 row) is carried
+        // by the mean profiles (kernel M), so the transforms see only the
+        // fluctuation and round relative to it
+        if (sizeof(T) == 4 && J.split && r == 0) v = czero<T>();
       }
     }
     lds[c * LDSN + fft_pad(x)] = v;
@@ -122,9 +126,92 @@ __global__ void __launch_bounds__(kNT) k_stepA(const JobDev<T>* __restrict__ job
 }
 
 // --------------------------------------------------------------------------
+// Kernel M (single precision): the x mean of each channel an x-inverse op
+// of the stage writes, as a function of z -- the z c2r of the channel's
+// kx = 0 coefficients -- into J.mean[op.ch * nz + n].  One block per job.
+// Kernel A leaves the kx = 0 row out of the channels, so kernel B's
+// products are formed from mean + fluctuation with the mean-times-mean part
+// left out: its x difference is zero, and in z it vanishes because uz and bz
+// have no x mean.  Without the split, an O(1) x-independent profile (a shear
+// layer, a current sheet) puts FP32 rounding of its own size into products
+// whose x differences should cancel exactly.
+// --------------------------------------------------------------------------
+template <typename T, int LOGN>
+__device__ __forceinline__ void stepM_body(const JobDev<T>& J, const StageDev& S, int pair,
+                                           int par, const cx<T>* __restrict__ tw, cx<T>* lds) {
+  constexpr int N = 1 << LOGN;
+  constexpr int P = 16;  // one slot per block: 256 threads x 16 points >= N
+  const int o0 = 2 * pair;
+  if (o0 >= S.nA) return;
+  const bool hb = o0 + 1 < S.nA;
+  const XinvOp& oa = S.a[o0];
+  const XinvOp& ob = S.a[hb ? o0 + 1 : o0];
+  const int nm = J.nm, rows = J.rows;
+  const float ffz = (float)J.fz, ffz2 = (float)J.fz2;
+  // the kx = 0 coefficient of channel op at kz = m (the values kernel A
+  // would put in row 0)
+  auto coef = [&](const XinvOp& op, int m) -> cx<T> {
+    const cx<T> f = J.state[op.from_new ? (par ^ 1) : par][op.src][(size_t)m * rows];
+    if (op.op == XOP_ID) return f;
+    if (op.op == XOP_UZ) return czero<T>();  // i kx psi with kx = 0
+    float lap = ffz2 * (float)(m * m);
+    if (m == 0) lap = 1.0f;
+    const T rl = __builtin_amdgcn_rcpf(lap);
+    const cx<T> ps = mk<T>(-f.x * rl, -f.y * rl);
+    const T yz = ffz * (float)m;
+    return mk<T>(yz * ps.y, -(yz * ps.x));
+  };
+  cx<T>* z = lds;
+  for (int k = threadIdx.x; k < nm; k += kNT) {
+    const cx<T> a = coef(oa, k);
+    const cx<T> b = hb ? coef(ob, k) : czero<T>();
+    if (k == 0) {
+      z[0] = mk<T>(a.x, b.x);
+    } else {
+      z[fft_pad(k)] = mk<T>(a.x - b.y, a.y + b.x);
+      z[fft_pad(N - k)] = mk<T>(a.x + b.y, b.x - a.y);
+    }
+  }
+  for (int k = nm + threadIdx.x; k <= N - nm; k += kNT) z[fft_pad(k)] = czero<T>();
+  __syncthreads();
+  fft_lds<T, LOGN, +1, kNT, P>(lds, 1, tw);
+  T* __restrict__ ma = J.mean + (size_t)oa.ch * N;
+  T* __restrict__ mb = J.mean + (size_t)ob.ch * N;
+  for (int n = threadIdx.x; n < N; n += kNT) {
+    const cx<T> zv = z[fft_pad(n)];
+    ma[n] = zv.x;
+    if (hb) mb[n] = zv.y;
+  }
+}
+
+// Grid: jobs x (kMeanCh / 2) op pairs.
+template <typename T>
+__global__ void __launch_bounds__(kNT) k_stepM(const JobDev<T>* __restrict__ jobs, int stage,
+                                               int step, const cx<T>* __restrict__ tw) {
+  extern __shared__ char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const JobDev<T>& J = jobs[blockIdx.x / (kMeanCh / 2)];
+  const int pair = blockIdx.x % (kMeanCh / 2);
+  if (stage >= J.nstage || !J.split) return;
+  const int par = (J.par0 + step) & 1;
+  switch (J.lognz) {
+#define X(L)                                                       \
+  case L:                                                          \
+    stepM_body<T, L>(J, J.st[stage], pair, par, tw, lds);          \
+    break;
+    X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+#undef X
+    default:
+      break;
+  }
+}
+
+// --------------------------------------------------------------------------
 // Kernel B: z pass, c2r of the channels, products, r2c of the products
 // --------------------------------------------------------------------------
-template <typename T, int LOGN, int V>
+// SPLIT (single precision): jobs with split carry mean profiles (kernel M)
+// and mean-free channels; a batch without such jobs runs SPLIT = 0.
+template <typename T, int LOGN, int V, int SPLIT>
 __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S, int local,
                                            int flags, const cx<T>* __restrict__ tw, cx<T>* lds) {
   using Pg = BProg<V>;
@@ -192,6 +279,9 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     const bool diag = (flags & kDiag) && J.loops % (flags >> kDiagShift) == 0;
     T mux = -INFINITY, muz = -INFINITY;  // NaN-propagating maxima, as np.max
     double ke = 0.0, ens = 0.0, flux = 0.0;
+    // single precision: the channels are fluctuations about their x means,
+    // whose z profiles kernel M wrote
+    const bool kSplit = SPLIT && J.split;
     for (int n = threadIdx.x; n < N; n += kNT) {
       T v[RB * NIN];
 #pragma unroll
@@ -203,17 +293,23 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
           v[q] = T(0);
         }
       }
+      T mv[NIN];
+#pragma unroll
+      for (int q = 0; q < NIN; ++q)
+        mv[q] = (kSplit && !Pg::zero_mean(q)) ? J.mean[(size_t)q * N + n] : T(0);
+      // the whole value of sequence q (row q / NIN, channel q % NIN)
+      auto whole = [&](int q) { return kSplit ? mv[q % NIN] + v[q] : v[q]; };
       if (diag) {
 #pragma unroll
         for (int r = 0; r < RB; ++r) {
           if (r < nrows) {
-            const T ux = v[r * NIN + Pg::UX], uz = v[r * NIN + Pg::UZ];
+            const T ux = whole(r * NIN + Pg::UX), uz = whole(r * NIN + Pg::UZ);
             mux = nan_max(mux, ux);
             muz = nan_max(muz, uz);
             ke += (double)uz * (double)uz + (double)ux * (double)ux;
-            const double w = (double)v[r * NIN];
+            const double w = (double)whole(r * NIN);
             ens += w * w;
-            if constexpr (Pg::FLUX >= 0) flux += (double)v[r * NIN + Pg::FLUX] * (double)uz;
+            if constexpr (Pg::FLUX >= 0) flux += (double)whole(r * NIN + Pg::FLUX) * (double)uz;
           }
         }
       }
@@ -223,7 +319,7 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
           if (q < nseq) {
 #pragma unroll
             for (int i = 0; i < kMaxPhys; ++i)
-              if (S.phys_ch[i] == q % NIN) J.phys[i][(size_t)(x0 + q / NIN) * N + n] = v[q];
+              if (S.phys_ch[i] == q % NIN) J.phys[i][(size_t)(x0 + q / NIN) * N + n] = whole(q);
           }
         }
       }
@@ -231,7 +327,10 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
       for (int r = 0; r < RB; ++r) {
         if (r < nrows) {
           T px[NEQ], pz[NEQ];
-          Pg::template products<T>(&v[r * NIN], px, pz);
+          if (kSplit)
+            Pg::template products_split<T>(mv, &v[r * NIN], px, pz);
+          else
+            Pg::template products<T>(&v[r * NIN], px, pz);
 #pragma unroll
           for (int e = 0; e < NEQ; ++e) lds[(r * NEQ + e) * LDSN + fft_pad(n)] = mk<T>(px[e], pz[e]);
         }
@@ -264,7 +363,7 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
   }
 }
 
-template <typename T, int V>
+template <typename T, int V, int SPLIT>
 __global__ void __launch_bounds__(kNT) k_stepB(const JobDev<T>* __restrict__ jobs,
                                                const int* __restrict__ pref, int njobs, int stage,
                                                int flags, const cx<T>* __restrict__ tw) {
@@ -278,7 +377,7 @@ __global__ void __launch_bounds__(kNT) k_stepB(const JobDev<T>* __restrict__ job
   switch (J.lognz) {
 #define X(L)                                                      \
   case L:                                                         \
-    stepB_body<T, L, V>(J, J.st[stage], local, flags, tw, lds);   \
+    stepB_body<T, L, V, SPLIT>(J, J.st[stage], local, flags, tw, lds); \
     break;
     X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 #undef X
@@ -619,9 +718,19 @@ template <typename T, int V>
 static void launchB(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int flags,
                     const cx<T>* tw, hipStream_t s) {
   if (bl.nB[stage][V] == 0) return;
-  MLV_SET_LDS_ONCE((k_stepB<T, V>), bl.ldsB[V]);
-  hipLaunchKernelGGL((k_stepB<T, V>), dim3(xcd_grid(bl.nB[stage][V])), dim3(kNT), bl.ldsB[V], s, jobs,
-                     (const int*)bl.prefB[stage][V], bl.njobs, stage, flags, tw);
+  const dim3 grid(xcd_grid(bl.nB[stage][V]));
+  const int* pref = bl.prefB[stage][V];
+  if constexpr (sizeof(T) == 4) {
+    if (bl.ldsM) {  // some job carries mean profiles
+      MLV_SET_LDS_ONCE((k_stepB<T, V, 1>), bl.ldsB[V]);
+      hipLaunchKernelGGL((k_stepB<T, V, 1>), grid, dim3(kNT), bl.ldsB[V], s, jobs, pref,
+                         bl.njobs, stage, flags, tw);
+      return;
+    }
+  }
+  MLV_SET_LDS_ONCE((k_stepB<T, V, 0>), bl.ldsB[V]);
+  hipLaunchKernelGGL((k_stepB<T, V, 0>), grid, dim3(kNT), bl.ldsB[V], s, jobs, pref, bl.njobs,
+                     stage, flags, tw);
 }
 
 template <typename T>
@@ -632,6 +741,13 @@ hipError_t fused_stage(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, 
     MLV_SET_LDS_ONCE(k_stepA<T>, bl.ldsA);
     hipLaunchKernelGGL(k_stepA<T>, dim3(xcd_grid(bl.nA[stage])), dim3(kNT), bl.ldsA, s, jobs,
                        (const int*)bl.prefA[stage], bl.njobs, stage, step, tw);
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (bl.ldsM) {
+      MLV_SET_LDS_ONCE(k_stepM<T>, bl.ldsM);
+      hipLaunchKernelGGL(k_stepM<T>, dim3(bl.njobs * (kMeanCh / 2)), dim3(kNT), bl.ldsM, s, jobs,
+                         stage, step, tw);
+    }
   }
   if (ev) (void)hipEventRecord(ev[1], s);
   launchB<T, BV_VORT1>(jobs, bl, stage, flags, tw, s);
@@ -662,13 +778,23 @@ hipError_t fused_prepare(const BatchLaunch& bl) {
   size_t b = 0;
   for (int v = 0; v < BV_COUNT; ++v) b = bl.ldsB[v] > b ? bl.ldsB[v] : b;
   if (bl.ldsA) MLV_SET_LDS_ONCE(k_stepA<T>, bl.ldsA);
+  if constexpr (sizeof(T) == 4) {
+    if (bl.ldsM) MLV_SET_LDS_ONCE(k_stepM<T>, bl.ldsM);
+  }
+
   if (bl.ldsCp[1]) MLV_SET_LDS_ONCE((k_stepC<T, 1>), bl.ldsCp[1]);
   if (bl.ldsCp[0]) MLV_SET_LDS_ONCE((k_stepC<T, 0>), bl.ldsCp[0]);
-  if (bl.ldsB[BV_VORT1]) MLV_SET_LDS_ONCE((k_stepB<T, BV_VORT1>), bl.ldsB[BV_VORT1]);
-  if (bl.ldsB[BV_VORT2]) MLV_SET_LDS_ONCE((k_stepB<T, BV_VORT2>), bl.ldsB[BV_VORT2]);
-  if (bl.ldsB[BV_VORT3]) MLV_SET_LDS_ONCE((k_stepB<T, BV_VORT3>), bl.ldsB[BV_VORT3]);
-  if (bl.ldsB[BV_MHD1]) MLV_SET_LDS_ONCE((k_stepB<T, BV_MHD1>), bl.ldsB[BV_MHD1]);
-  if (bl.ldsB[BV_MHD2]) MLV_SET_LDS_ONCE((k_stepB<T, BV_MHD2>), bl.ldsB[BV_MHD2]);
+#define MLV_B_LDS(V)                                                           \
+  if (bl.ldsB[V]) {                                                            \
+    MLV_SET_LDS_ONCE((k_stepB<T, V, 0>), bl.ldsB[V]);                          \
+    if constexpr (sizeof(T) == 4) MLV_SET_LDS_ONCE((k_stepB<T, V, 1>), bl.ldsB[V]); \
+  }
+  MLV_B_LDS(BV_VORT1)
+  MLV_B_LDS(BV_VORT2)
+  MLV_B_LDS(BV_VORT3)
+  MLV_B_LDS(BV_MHD1)
+  MLV_B_LDS(BV_MHD2)
+#undef MLV_B_LDS
   (void)b;
   return hipGetLastError();
 }

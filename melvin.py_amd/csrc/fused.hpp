@@ -76,6 +76,11 @@ struct JobDev {
   // has to resolve a fraction of one FP32 ulp, so 8 bits of it suffice
   uint32_t* comp[kMaxFields];
   cx<T>* work;
+  // single precision with split = 1: each z-pass input channel's x mean as
+  // a function of z (kMeanCh rows of nz values, in the workspace after the
+  // channels); the x passes then carry only the fluctuations
+  int split;
+  T* mean;
   cx<T>* psi;
   T* phys[kMaxPhys];
   StageDev st[2];
@@ -101,19 +106,38 @@ __device__ __forceinline__ uint32_t bf2_pack(float re, float im) {
   return bf16_bits(re) | (bf16_bits(im) << 16);
 }
 
-// Per-variant channel/product description used by kernel B.
+// a*b without its mean-times-mean part: (am + af)(bm + bf) - am bm, with
+// means m and fluctuations f (single-precision mean split, see kernel M)
+template <typename T>
+__device__ __forceinline__ T bil(T am, T af, T bm, T bf) {
+  return am * bf + af * bm + af * bf;
+}
+
+// Per-variant channel/product description used by kernel B.  products()
+// takes whole values; products_split() takes the x means m and the
+// fluctuations f of the channels and leaves out the mean-times-mean part.
 template <int V> struct BProg;
 template <int NF> struct BVortProg {
   static constexpr int NIN = NF + 2;  // fields, ux, uz
   static constexpr int NEQ = NF;
   static constexpr int UX = NF, UZ = NF + 1;  // velocity channels
   static constexpr int FLUX = NF >= 2 ? 1 : -1;  // scalar whose flux f*uz is tracked
+  // channels whose x mean is zero (d/dx of a potential)
+  static constexpr bool zero_mean(int q) { return q == UZ; }
   template <typename T>
   __device__ __forceinline__ static void products(const T* v, T* px, T* pz) {
 #pragma unroll
     for (int e = 0; e < NF; ++e) {
       px[e] = -(v[NF] * v[e]);
       pz[e] = -(v[NF + 1] * v[e]);
+    }
+  }
+  template <typename T>
+  __device__ __forceinline__ static void products_split(const T* m, const T* f, T* px, T* pz) {
+#pragma unroll
+    for (int e = 0; e < NF; ++e) {
+      px[e] = -bil(m[NF], f[NF], m[e], f[e]);
+      pz[e] = -bil(m[NF + 1], f[NF + 1], m[e], f[e]);
     }
   }
 };
@@ -125,22 +149,34 @@ template <> struct BProg<BV_MHD1> {
   static constexpr int NIN = 6;
   static constexpr int UX = 2, UZ = 3;
   static constexpr int FLUX = -1;
+  static constexpr bool zero_mean(int q) { return q == 3 || q == 5; }  // uz, bz
   static constexpr int NEQ = 1;
   template <typename T>
   __device__ __forceinline__ static void products(const T* v, T* px, T* pz) {
     px[0] = -(v[2] * v[0]) + v[4] * v[1];
     pz[0] = -(v[3] * v[0]) + v[5] * v[1];
   }
+  template <typename T>
+  __device__ __forceinline__ static void products_split(const T* m, const T* f, T* px, T* pz) {
+    px[0] = -bil(m[2], f[2], m[0], f[0]) + bil(m[4], f[4], m[1], f[1]);
+    pz[0] = -bil(m[3], f[3], m[0], f[0]) + bil(m[5], f[5], m[1], f[1]);
+  }
 };
 template <> struct BProg<BV_MHD2> {
   static constexpr int NIN = 6;
   static constexpr int UX = 2, UZ = 3;
   static constexpr int FLUX = -1;
+  static constexpr bool zero_mean(int q) { return q == 3 || q == 5; }  // uz, bz
   static constexpr int NEQ = 1;
   template <typename T>
   __device__ __forceinline__ static void products(const T* v, T* px, T* pz) {
     px[0] = -(v[2] * v[1]) + v[4] * v[0];
     pz[0] = -(v[3] * v[1]) + v[5] * v[0];
+  }
+  template <typename T>
+  __device__ __forceinline__ static void products_split(const T* m, const T* f, T* px, T* pz) {
+    px[0] = -bil(m[2], f[2], m[1], f[1]) + bil(m[4], f[4], m[0], f[0]);
+    pz[0] = -bil(m[3], f[3], m[1], f[1]) + bil(m[5], f[5], m[0], f[0]);
   }
 };
 
@@ -204,11 +240,21 @@ __host__ __device__ constexpr int bvar_neq(int v) { return v <= BV_VORT3 ? v + 1
 // z-pass inputs occupy channels [0, base)
 __host__ __device__ constexpr int pchan(int base, int e, int k) { return base + 2 * e + k; }
 
+// Rows of the mean-profile area (the most z-pass input channels).
+constexpr int kMeanCh = 6;
+// Workspace of a job, in complex elements: the z-pass inputs and the Px/Pz
+// outputs, (nin + 2 neq) x (nx, nm), then the mean profiles.
+inline size_t fused_work_elems(int nin, int neq, int nx, int nz) {
+  const int nm = (nz - 1) / 3;
+  return (size_t)(nin + 2 * neq) * nx * nm + (size_t)(kMeanCh / 2) * nz;
+}
+
 struct BatchLaunch {
   int njobs;
   int wideC;                           // points policy of kernel C (fused_points)
   int nA[2], nB[2][BV_COUNT], nC[2];  // total blocks per launch
   size_t ldsA, ldsB[BV_COUNT], ldsC;
+  size_t ldsM;                         // single precision mean profiles: one slot; 0 = none
   int* prefA[2];                       // device prefix arrays (njobs + 1)
   int* prefB[2][BV_COUNT];
   int* prefC[2];

@@ -47,10 +47,14 @@ class Job:
 
     def __init__(self, problem, nx, nz, lx, lz, dtype="f32", dt=1e-3, coeff=(0.0,),
                  param=(), nfield=None, ab_order=2, fd_order=2, alpha=0.51,
-                 want_psi=False, want_phys=False, compensated=True, device=None):
+                 want_psi=False, want_phys=False, compensated=True, mean_split=True,
+                 device=None):
         if problem not in PROBLEMS:
             raise ValueError(f"unknown problem {problem!r}")
         self.problem = problem
+        # single precision: transform each channel's x mean apart from its
+        # fluctuation (mlv_job_desc.mean_split)
+        self.mean_split = bool(mean_split)
         self.nfield = FIELDS.get(problem, nfield if nfield is not None else len(coeff))
         if problem == "vorticity" and not 1 <= self.nfield <= 3:
             raise ValueError("vorticity jobs carry 1..3 fields")
@@ -184,6 +188,7 @@ class Job:
                 d.phys[i] = p.data_ptr()
         d.curr, d.parity = self.curr, self.parity
         d.t, d.loops = self.t, self.steps
+        d.mean_split = 1 if self.mean_split else 0
         return d
 
 

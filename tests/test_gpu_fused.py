@@ -255,3 +255,36 @@ def test_autotune_steps_are_real_steps(mlv, dtype):
         assert t.steps == p.steps == 10
         for x, y in zip(t.state(), p.state()):
             np.testing.assert_array_equal(x, y)
+
+
+def test_fp32_mean_split_keeps_equilibrium_accuracy(mlv):
+    """Resistive tearing (an O(10) current sheet with 1e-4 vorticity noise,
+    whose Lorentz force cancels exactly): with the x-mean split the FP32 run
+    stays within 1e-5 of FP64 in omega and j; transforming the channels
+    whole, FP32 rounding of the sheet's products swamps omega.  A job without
+    the split gives the same bits alone and batched beside a split job."""
+    S = mlv.serving
+    req = S.Request(0, 0.0, "mhd", 128, 120, seed=3)
+
+    def run(dtype, split, partner=None):
+        job = S.make_job(req, dtype)
+        job.mean_split = split
+        jobs = [job] + ([partner] if partner is not None else [])
+        b = mlv.fused.Batch(jobs)
+        b.step(req.steps)
+        b.close()
+        return job.state()
+
+    ref = run("f64", False)
+    on = run("f32", True)
+    off = run("f32", False)
+    e_on = [rel(a, b) for a, b in zip(on, ref)]
+    e_off = [rel(a, b) for a, b in zip(off, ref)]
+    print(f"[fused] tearing 128^2 FP32 vs FP64 (omega, j): split {e_on}, whole {e_off}", flush=True)
+    assert e_on[0] < 1e-5 and e_on[1] < 1e-6
+    assert e_off[0] > 100 * e_on[0]
+    partner = S.make_job(S.Request(1, 0.0, "vortex", 64, 10, seed=4), "f32")
+    assert partner.mean_split
+    mixed = run("f32", False, partner)
+    for x, y in zip(mixed, off):
+        np.testing.assert_array_equal(x, y)
