@@ -93,11 +93,14 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
   }
   __syncthreads();
   fft_lds<T, LOGN, +1, kNT, P>(lds, ncol, tw);
+  // x % 4 fastest, then the column: each group of 16 lanes stores one
+  // 4-row group's TC columns, contiguous in the interleaved layout
 #pragma unroll 8
   for (int i = 0; i < ITER; ++i) {
     const int e = threadIdx.x + i * kNT;
-    const int c = e % TC, x = e / TC;
-    if (c < ncol) dst[(size_t)x * nm + c0 + c] = lds[c * LDSN + fft_pad(x)];
+    const int g = e >> 2;
+    const int c = g % TC, x = (g / TC) * 4 + (e & 3);
+    if (c < ncol) dst[xrow_base(x, nm) + (size_t)(c0 + c) * 4] = lds[c * LDSN + fft_pad(x)];
   }
 }
 
@@ -244,9 +247,9 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     for (int s = wv / WPS; s < nslot; s += NW / WPS) {
       cx<T>* z = lds + s * LDSN;
       const int qa = 2 * s, qb = qa + 1;
-      const cx<T>* __restrict__ pa = work + (size_t)(qa % NIN) * chs + (size_t)(x0 + qa / NIN) * nm;
+      const cx<T>* __restrict__ pa = work + (size_t)(qa % NIN) * chs + xrow_base(x0 + qa / NIN, nm);
       const cx<T>* __restrict__ pb =
-          qb < nseq ? work + (size_t)(qb % NIN) * chs + (size_t)(x0 + qb / NIN) * nm : nullptr;
+          qb < nseq ? work + (size_t)(qb % NIN) * chs + xrow_base(x0 + qb / NIN, nm) : nullptr;
       // loads of up to KB iterations in flight before any is used
 #pragma unroll 1
       for (int i0 = ssub * KB; i0 < KIT; i0 += WPS * KB) {
@@ -254,8 +257,8 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
 #pragma unroll
         for (int i = 0; i < KB; ++i) {
           const int k = lane + 64 * (i0 + i);
-          av[i] = k < nm ? pa[k] : czero<T>();
-          bv[i] = (pb && k < nm) ? pb[k] : czero<T>();
+          av[i] = k < nm ? pa[(size_t)k * 4] : czero<T>();
+          bv[i] = (pb && k < nm) ? pb[(size_t)k * 4] : czero<T>();
         }
 #pragma unroll
         for (int i = 0; i < KB; ++i) {

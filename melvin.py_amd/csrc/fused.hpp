@@ -240,6 +240,14 @@ __host__ __device__ constexpr int bvar_neq(int v) { return v <= BV_VORT3 ? v + 1
 // z-pass inputs occupy channels [0, base)
 __host__ __device__ constexpr int pchan(int base, int e, int k) { return base + 2 * e + k; }
 
+// Layout of the z-pass input channels (kernel A writes them, kernel B reads
+// them): x rows in groups of four, interleaved by kz, so that the four
+// columns of one of A's tiles fill whole 128-byte lines.  Element (x, k) of
+// a channel is at xrow_base(x, nm) + 4 k.
+__host__ __device__ constexpr size_t xrow_base(int x, int nm) {
+  return (size_t)(x >> 2) * (4 * (size_t)nm) + (size_t)(x & 3);
+}
+
 // Rows of the mean-profile area (the most z-pass input channels).
 constexpr int kMeanCh = 6;
 // Workspace of a job, in complex elements: the z-pass inputs and the Px/Pz
