@@ -220,8 +220,9 @@ int mlv_batch_graph_step(mlv_batch* batch, int nsteps, void* stream);
 int mlv_batch_set_dt(mlv_batch* batch, int job, double dt, void* stream);
 /* Pick the update kernel's tile shape for this batch by measurement: runs
  * nsteps real steps with each shape (both compute bit-identical results, so
- * the steps advance the simulations as mlv_batch_step would), keeps the
- * faster and remembers it for batches created later for the same set of job
+ * the steps advance the simulations as mlv_batch_step would; they run in
+ * the shape order narrow, wide, wide, narrow, half of nsteps each, and a
+ * shape's figure is its faster half), keeps the faster and remembers it for batches created later for the same set of job
  * shapes (process-wide).  *chosen = 1 for the wide shape (32 points per
  * thread where it applies), 0 for the narrow one.  Synchronises. */
 int mlv_batch_autotune(mlv_batch* batch, int nsteps, void* stream, int* chosen);

@@ -1093,21 +1093,35 @@ int mlv_batch_autotune(mlv_batch* bp, int nsteps, void* stream, int* chosen) {
   hipEvent_t ev[2];
   HIP_TRY(hipEventCreate(&ev[0]), "hipEventCreate");
   HIP_TRY(hipEventCreate(&ev[1]), "hipEventCreate");
-  float ms[2] = {0.f, 0.f};
   hipError_t e = hipSuccess;
-  for (int wide = 0; wide < 2 && e == hipSuccess; ++wide) {
-    select_c_policy(b->bl, wide);
+  // Segments in the order narrow, wide, wide, narrow (half of nsteps each;
+  // one segment per shape when nsteps is 1), each timed on its own; a shape's
+  // figure is its faster segment per step.  The mirrored order and the
+  // minimum keep a clock ramp after an idle period, and the one-time cost of
+  // a kernel's first launch, from deciding the choice.
+  const int h1 = nsteps >= 2 ? nsteps / 2 : nsteps, h2 = nsteps - h1;
+  const int seg_shape[4] = {0, 1, 1, 0};
+  const int seg_steps[4] = {h1, h1, h2, h2};
+  float best_ms[2] = {INFINITY, INFINITY};
+  for (int k = 0; k < 4 && e == hipSuccess; ++k) {
+    const int n = seg_steps[k];
+    if (n == 0) continue;
+    select_c_policy(b->bl, seg_shape[k]);
+    float t = 0.f;
     e = hipEventRecord(ev[0], s);
-    if (e == hipSuccess) e = enqueue_steps(b, nsteps, 0, b->steps, s, false);
+    if (e == hipSuccess) e = enqueue_steps(b, n, 0, b->steps, s, false);
     if (e == hipSuccess) e = hipEventRecord(ev[1], s);
     if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms[wide], ev[0], ev[1]);
-    if (e == hipSuccess) b->steps += nsteps;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, ev[0], ev[1]);
+    if (e == hipSuccess) {
+      b->steps += n;
+      best_ms[seg_shape[k]] = std::min(best_ms[seg_shape[k]], t / n);
+    }
   }
   (void)hipEventDestroy(ev[0]);
   (void)hipEventDestroy(ev[1]);
   if (e != hipSuccess) return hip_fail(e, "autotune");
-  const int best = ms[1] < ms[0] ? 1 : 0;
+  const int best = best_ms[1] < best_ms[0] ? 1 : 0;
   select_c_policy(b->bl, best);
   // a captured graph holds the launches of the shape it was captured with
   if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
