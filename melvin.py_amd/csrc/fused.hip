@@ -93,14 +93,14 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
   }
   __syncthreads();
   fft_lds<T, LOGN, +1, kNT, P>(lds, ncol, tw);
-  // x % 4 fastest, then the column: each group of 16 lanes stores one
-  // 4-row group's TC columns, contiguous in the interleaved layout
+  // x % kXil fastest, then the column: consecutive lanes store one row
+  // group's TC columns, contiguous in the interleaved layout
 #pragma unroll 8
   for (int i = 0; i < ITER; ++i) {
     const int e = threadIdx.x + i * kNT;
-    const int g = e >> 2;
-    const int c = g % TC, x = (g / TC) * 4 + (e & 3);
-    if (c < ncol) dst[xrow_base(x, nm) + (size_t)(c0 + c) * 4] = lds[c * LDSN + fft_pad(x)];
+    const int g = e / kXil;
+    const int c = g % TC, x = (g / TC) * kXil + e % kXil;
+    if (c < ncol) dst[xrow_base(x, nm) + (size_t)(c0 + c) * kXil] = lds[c * LDSN + fft_pad(x)];
   }
 }
 
@@ -257,8 +257,8 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
 #pragma unroll
         for (int i = 0; i < KB; ++i) {
           const int k = lane + 64 * (i0 + i);
-          av[i] = k < nm ? pa[(size_t)k * 4] : czero<T>();
-          bv[i] = (pb && k < nm) ? pb[(size_t)k * 4] : czero<T>();
+          av[i] = k < nm ? pa[(size_t)k * kXil] : czero<T>();
+          bv[i] = (pb && k < nm) ? pb[(size_t)k * kXil] : czero<T>();
         }
 #pragma unroll
         for (int i = 0; i < KB; ++i) {

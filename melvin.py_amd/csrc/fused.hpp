@@ -241,11 +241,14 @@ __host__ __device__ constexpr int bvar_neq(int v) { return v <= BV_VORT3 ? v + 1
 __host__ __device__ constexpr int pchan(int base, int e, int k) { return base + 2 * e + k; }
 
 // Layout of the z-pass input channels (kernel A writes them, kernel B reads
-// them): x rows in groups of four, interleaved by kz, so that the four
-// columns of one of A's tiles fill whole 128-byte lines.  Element (x, k) of
-// a channel is at xrow_base(x, nm) + 4 k.
+// them): x rows in groups of kXil, interleaved by kz, so that one of A's
+// column tiles stores runs of kXil x TC elements rather than TC.  Element
+// (x, k) of a channel is at xrow_base(x, nm) + kXil k.  Groups of two beat
+// four: A's runs are 64 bytes instead of 128 (+10 us at 64 x 1024^2), but a
+// z-pass block reads both rows of each group (-32 us).
+constexpr int kXil = 2;
 __host__ __device__ constexpr size_t xrow_base(int x, int nm) {
-  return (size_t)(x >> 2) * (4 * (size_t)nm) + (size_t)(x & 3);
+  return (size_t)(x / kXil) * (kXil * (size_t)nm) + (size_t)(x % kXil);
 }
 
 // Rows of the mean-profile area (the most z-pass input channels).
