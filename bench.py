@@ -313,7 +313,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--batch", type=int, default=64,
                     help="independent 1024^2 simulations advanced together (about 50 MB each; "
-                         "throughput at 16/24/32/48/64: 4.29/4.37/4.52/4.56/4.63e10)")
+                         "throughput still rises slowly from 32 to 64)")
     ap.add_argument("--fp64-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
