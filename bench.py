@@ -42,12 +42,10 @@ DT = 2e-4
 Q = 4
 
 
-def kh_initial(seed):
-    from oracle import melvin_oracle as O
-
-    p = O.make_params(dict(nx=NX, nz=NZ, lx=LX, lz=LZ, initial_dt=DT, integrator_order=Q))
-    w, c = O.ic_kelvin_helmholtz(p, seed=seed, amp=1e-3)
-    return p, O.to_spectral(p, w), O.to_spectral(p, c)
+def kh_initial(m, seed):
+    """Spectral omega and scalar of the Kelvin-Helmholtz start (the
+    package's initial-condition generator; host FP64)."""
+    return m.initial.kelvin_helmholtz(NX, seed=seed, amp=1e-3, length=LX)
 
 
 def kernel_bytes(nx, nz, nf, q, cbytes, carry=False):
@@ -85,7 +83,9 @@ def measured_traffic(kernel, batch):
 
 
 def cpu_baseline(seconds=12.0):
-    """The oracle's FP64 KH step (NumPy/pocketfft, one thread) on the host."""
+    """The oracle's FP64 KH step (NumPy/pocketfft, one thread) on one pinned
+    host core: 3 warm-up steps, then steps until ``seconds`` have passed;
+    the rate is from the median step time."""
     from oracle import melvin_oracle as O
 
     try:
@@ -94,23 +94,29 @@ def cpu_baseline(seconds=12.0):
     except (AttributeError, OSError):
         old = None
     try:
-        p, w, c = kh_initial(0)
+        p = O.make_params(dict(nx=NX, nz=NZ, lx=LX, lz=LZ, initial_dt=DT, integrator_order=Q))
+        w, c = O.ic_kelvin_helmholtz(p, seed=0, amp=1e-3)
         st = O.new_state(p, ("w", "c"), dt=DT)
-        st["w"], st["c"] = w, c
+        st["w"], st["c"] = O.to_spectral(p, w), O.to_spectral(p, c)
         body = lambda p, s: O.step_vorticity(p, s, 1.0 / RE, ("c",))  # noqa: E731
-        body(p, st)
-        n = 0
+        for _ in range(3):
+            body(p, st)
+        times = []
         t0 = time.perf_counter()
         while True:
+            t1 = time.perf_counter()
             body(p, st)
-            n += 1
+            times.append(time.perf_counter() - t1)
             el = time.perf_counter() - t0
-            if el >= seconds and n >= 3:
+            if el >= seconds and len(times) >= 3:
                 break
-        return {"value": NX * NZ * n / el, "unit": "grid-pt-steps/s", "cores": 1,
+        med = float(np.median(times))
+        return {"value": NX * NZ / med, "unit": "grid-pt-steps/s", "cores": 1,
                 "kind": "port",
+                "affinity_cpus_available": len(old) if old is not None else None,
                 "sample": f"NumPy FP64 oracle, Kelvin-Helmholtz 1024^2 omega+scalar AB4, "
-                          f"{n} steps after 1 warm-up ({el:.1f} s), 1 simulation, one core"}
+                          f"3 warm-up steps then {len(times)} timed steps ({el:.1f} s), "
+                          f"median {med * 1e3:.1f} ms/step, 1 simulation, pinned to one core"}
     finally:
         if old is not None:
             os.sched_setaffinity(0, old)
@@ -357,7 +363,7 @@ def main():
     def make_jobs(n, dtype, mean_split=False):
         jobs = []
         for s in range(n):
-            _, w, c = kh_initial(s)
+            w, c = kh_initial(m, s)
             j = fused.Job("vorticity", NX, NZ, LX, LZ, dtype=dtype, dt=DT,
                           coeff=[1.0 / RE, 1.0 / RE], ab_order=Q, mean_split=mean_split)
             j.set_state([w, c])

@@ -96,14 +96,31 @@ def vortex_pair(n, seed=0, radius=0.1, separation=0.25, length=1.0):
 
 def tearing(n, seed=0, width=0.1, amp=1e-4, length=1.0):
     """Resistive tearing: a periodic double Harris sheet
-    Bx = tanh((z-L/4)/a) - tanh((z-3L/4)/a) - 1 with current j = dBx/dz (the
-    reference evolves j, resistive_tearing_instability.py:180-204), omega = 0
-    plus seeded noise of amplitude ``amp``.  Returns (omega, j) spectra."""
+    Bx = tanh((z-L/4)/a) - tanh((z-3L/4)/a) (summed over periodic images)
+    with current j = dBx/dz (the reference evolves j,
+    resistive_tearing_instability.py:36-60), omega = 0 plus seeded noise of
+    amplitude ``amp``.  Returns (omega, j) spectra."""
     X, Z = grid(n, n, length, length)
-    z1, z2 = 0.25 * length, 0.75 * length
-    j = (1 / np.cosh((Z - z1) / width) ** 2 - 1 / np.cosh((Z - z2) / width) ** 2) / width
     w = noise((n, n), amp, seed)
-    return spectral(w), spectral(j)
+    return spectral(w), spectral(_sheet_current(Z, width, length, np))
+
+
+# Periodic images each side of the domain summed into the double sheet:
+# sheets of width a at L/4 and 3L/4 leave sech^2(L/(4a)) at the domain edge
+# (2.7e-2 of the peak at a = L/10), which a single image would turn into a
+# jump of j there; with five images the remainder is below sech^2(4.5 L/a).
+_IMAGES = 2
+
+
+def _sheet_current(Z, width, length, mod):
+    """j = dBx/dz of the periodic double Harris sheet, Bx = sum over images
+    of tanh((z - L/4 - kL)/a) - tanh((z - 3L/4 - kL)/a) (``mod`` is numpy
+    or torch)."""
+    j = 0.0 * Z
+    for k in range(-_IMAGES, _IMAGES + 1):
+        z1, z2 = (0.25 + k) * length, (0.75 + k) * length
+        j = j + (1 / mod.cosh((Z - z1) / width) ** 2 - 1 / mod.cosh((Z - z2) / width) ** 2) / width
+    return j
 
 
 def ddc_noise(n, length, seed=0, amp=1e-3):
@@ -169,8 +186,7 @@ def tearing_device(n, seed=0, width=0.1, amp=1e-4, length=1.0, double=False):
     import torch
 
     X, Z = _device_grid(n, length, torch.float64)
-    z1, z2 = 0.25 * length, 0.75 * length
-    j = (1 / torch.cosh((Z - z1) / width) ** 2 - 1 / torch.cosh((Z - z2) / width) ** 2) / width
+    j = _sheet_current(Z, width, length, torch)
     g = torch.Generator(device="cuda")
     g.manual_seed(int(seed))
     w = amp * (2 * torch.rand((n, n), generator=g, device="cuda", dtype=torch.float64) - 1)

@@ -1,0 +1,238 @@
+"""The fused kernels at the sizes the benchmark configurations run: 512^2
+and 1024^2 fully spectral grids (nine and ten radix-2 levels along both
+axes, the single-precision update kernel's 16-point tiles, the interleaved
+z-pass channel layout) and the 1024 x 513 Fourier-x / FD-z grid.
+
+FP64 is checked against the oracle at <= 1e-12.  FP32 is checked against
+the oracle directly (so a bug shared by both precisions' kernels cannot
+hide behind an FP32-vs-FP64 comparison) and against FP64 over the
+configuration's step count at the stated <= 1e-5 relative L2 in omega."""
+import numpy as np
+import pytest
+
+from conftest import rel
+from oracle import melvin_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RE = 1e4
+KH_DT = 2e-4
+
+
+def _kh_params(n):
+    return O.make_params(dict(nx=n, nz=n, lx=2.0, lz=2.0, initial_dt=KH_DT, integrator_order=4))
+
+
+def _kh_start(p, seed):
+    w, c = O.ic_kelvin_helmholtz(p, seed=seed, amp=1e-3)
+    return O.to_spectral(p, w), O.to_spectral(p, c)
+
+
+def _kh_job(mlv, p, dtype, start, **kw):
+    j = mlv.fused.Job("vorticity", p.nx, p.nz, p.lx, p.lz, dtype=dtype, dt=KH_DT,
+                      coeff=[1.0 / RE, 1.0 / RE], ab_order=4, **kw)
+    j.set_state(list(start))
+    return j
+
+
+def _kh_oracle(p, start, n):
+    st = O.new_state(p, ("w", "c"), dt=KH_DT)
+    st["w"], st["c"] = start[0].copy(), start[1].copy()
+    for _ in range(n):
+        O.step_vorticity(p, st, 1.0 / RE, ("c",))
+    return st
+
+
+@pytest.mark.parametrize("n", [512, 1024])
+def test_kh_fp64_vs_oracle(mlv, n):
+    """Kelvin-Helmholtz omega + passive scalar, AB4 semi-implicit (the body of
+    kelvin_helmholtz_instability.py:117-129 with its ink block), 5 steps."""
+    p = _kh_params(n)
+    start = _kh_start(p, seed=1)
+    st = _kh_oracle(p, start, 5)
+    print(f"[headline] KH {n}^2 FP64 x5 vs oracle", flush=True)
+    job = _kh_job(mlv, p, "f64", start, want_psi=True, want_phys=True)
+    b = mlv.fused.Batch([job])
+    b.step(4)
+    b.step(1, write_phys=True)
+    w, c = job.state()
+    assert rel(w, st["w"]) <= 1e-12
+    assert rel(c, st["c"]) <= 1e-12
+    assert rel(job.psi_spectral(), st["psi"]) <= 1e-12
+    # the physical channels the z pass wrote on the last step (its input state)
+    w_phys = job.physical()[0]
+    prev = _kh_oracle(p, start, 4)
+    assert rel(w_phys, O.to_physical(p, prev["w"])) <= 1e-12
+    b.close()
+
+
+@pytest.mark.parametrize("split", [False, True])
+@pytest.mark.parametrize("n", [512, 1024])
+def test_kh_fp32_vs_oracle(mlv, n, split):
+    """The single-precision kernels (narrow update tiles, with and without
+    the x-mean split) against the FP64 oracle after 5 steps, in a batch of
+    two simulations so that the job table holds more than one entry."""
+    p = _kh_params(n)
+    start = _kh_start(p, seed=2)
+    st = _kh_oracle(p, start, 5)
+    print(f"[headline] KH {n}^2 FP32 split={split} x5 vs oracle", flush=True)
+    job = _kh_job(mlv, p, "f32", start, mean_split=split)
+    other = _kh_job(mlv, p, "f32", _kh_start(p, seed=3), mean_split=split)
+    b = mlv.fused.Batch([other, job])
+    b.step(5)
+    w, c = job.state()
+    ew, ec = rel(w, st["w"]), rel(c, st["c"])
+    print(f"[headline]   omega {ew:.2e} scalar {ec:.2e}", flush=True)
+    assert ew <= 1e-5 and ec <= 1e-5
+    b.close()
+
+
+def test_kh_1024_fp32_vs_fp64_1000_steps(mlv):
+    """The precision target of the Kelvin-Helmholtz configuration: 1024^2,
+    1000 steps, FP32 (batched, channels transformed whole as the benchmark
+    runs it; and with the library's default x-mean split) against the plain
+    FP64 run from the same start: omega relative L2 <= 1e-5.  The FP32 batch
+    picks its update tile shape with the autotuner first, whose steps are
+    real steps of both shapes."""
+    p = _kh_params(1024)
+    start = _kh_start(p, seed=0)
+    nsteps = 1000
+    ref = _kh_job(mlv, p, "f64", start)
+    b = mlv.fused.Batch([ref])
+    b.step(nsteps)
+    w64, c64 = ref.state()
+    b.close()
+    for split in (False, True):
+        print(f"[headline] KH 1024^2 FP32 split={split} x{nsteps} vs FP64", flush=True)
+        job = _kh_job(mlv, p, "f32", start, mean_split=split)
+        other = _kh_job(mlv, p, "f32", _kh_start(p, seed=5), mean_split=split)
+        b = mlv.fused.Batch([job, other])
+        tune = 3
+        b.autotune(steps=tune)
+        b.step(nsteps - 2 * tune)
+        w32, c32 = job.state()
+        ew, ec = rel(w32, w64), rel(c32, c64)
+        print(f"[headline]   omega {ew:.2e} scalar {ec:.2e}", flush=True)
+        assert job.steps == nsteps
+        assert ew <= 1e-5 and ec <= 1e-5
+        b.close()
+
+
+def _rbc_params(nx=1024, nz=513):
+    return O.make_params(dict(nx=nx, nz=nz, lx=2.0, lz=1.0, Pr=1.0, Ra=1e7, alpha=0.51,
+                              integrator_order=2, initial_dt=1e-5,
+                              discretisation=["spectral", "fdm"]))
+
+
+def _rbc_start(p, seed=0):
+    _, Z = O.grid(p)
+    t = 1.0 - Z / p.lz + O.noise((p.nx, p.nz), 1e-3, seed)
+    w = O.noise((p.nx, p.nz), 1e-3, seed + 1)
+    return O.to_spectral(p, w), O.to_spectral(p, t)
+
+
+def _rbc_job(mlv, p, dtype, start, **kw):
+    j = mlv.fdz.FdzJob(p.nx, p.nz, p.lx, p.lz, p.Pr, p.Ra, dtype=dtype, dt=p.initial_dt,
+                       scheme="pc", ab_order=2, fd_order=2, alpha=p.alpha, **kw)
+    j.set_state(list(start))
+    return j
+
+
+def test_rbc_1024x513_pc_fp64_vs_oracle(mlv):
+    """Rayleigh-Benard at the configuration's grid (1024 x 513, Ra = 1e7),
+    the AB2/AM2 predictor-corrector with theta-implicit z diffusion, FP64,
+    5 steps against the oracle's restatement."""
+    p = _rbc_params()
+    start = _rbc_start(p)
+    st = O.new_state(p, ("w", "tmp"), dt=p.initial_dt)
+    st["w"], st["tmp"] = start[0].copy(), start[1].copy()
+    for _ in range(5):
+        O.step_rbc_pc(p, st)
+    print("[headline] RBC 1024x513 PC FP64 x5 vs oracle", flush=True)
+    job = _rbc_job(mlv, p, "f64", start, want_psi=True)
+    b = mlv.fdz.FdzBatch([job])
+    b.step(5)
+    w, t = job.state()
+    assert rel(w, st["w"]) <= 1e-12
+    assert rel(t, st["tmp"]) <= 1e-12
+    assert rel(job.step_psi(), st["psi"]) <= 1e-12
+    b.close()
+
+
+def test_rbc_1024x513_fp32_vs_fp64_2000_steps(mlv):
+    """The configuration's 2000 steps from dt = 1e-5 with the reference's
+    CFL ticker run on the device: FP32 transforms and state (FP64 solves)
+    against FP64, omega and T relative L2 <= 1e-5, same final dt."""
+    p = _rbc_params()
+    w0, t0 = _rbc_start(p, seed=4)
+    out = {}
+    for dtype in ("f64", "f32"):
+        print(f"[headline] RBC 1024x513 PC {dtype} x2000, device CFL ticker", flush=True)
+        job = _rbc_job(mlv, p, dtype, (w0, t0))
+        b = mlv.fdz.FdzBatch([job])
+        mlv.fdz.run_device(b, 2000)
+        out[dtype] = (job.state(), job.dt)
+        b.close()
+    ew = rel(out["f32"][0][0], out["f64"][0][0])
+    et = rel(out["f32"][0][1], out["f64"][0][1])
+    print(f"[headline]   omega {ew:.2e} T {et:.2e} dt {out['f32'][1]:.3e} / {out['f64'][1]:.3e}",
+          flush=True)
+    assert out["f32"][1] == out["f64"][1]
+    assert ew <= 1e-5 and et <= 1e-5
+
+
+DDC_L = 1e5 ** 0.25
+
+
+def _ddc_job(mlv, n, r0, seed, dtype):
+    from melvin_amd import initial
+
+    j = mlv.fused.Job("ddc", n, n, DDC_L, DDC_L, dtype=dtype, dt=1e-3,
+                      coeff=[7.0, 1.0, 0.01], param=[7.0, r0])
+    j.set_state(list(initial.ddc_noise(n, DDC_L, seed=seed)))
+    return j
+
+
+def test_ddc_mixed_graph_batch_fp32_vs_fp64(mlv):
+    """The batching configuration's form: double-diffusive jobs on grids from
+    {128, 256, 512, 1024}^2 with their own density ratios, FP32, advanced by
+    a captured HIP graph.  Each job equals the same FP32 job run alone with
+    eager steps (bit for bit), is within 1e-5 of its FP64 run in omega, T
+    and S, and the FP64 run of the smallest grid matches the oracle."""
+    sizes = [128, 1024, 256, 512, 128, 256]
+    r0s = [1.2, 1.9, 1.5, 1.35, 2.0, 1.7]
+    nsteps = 40
+    jobs = [_ddc_job(mlv, n, r0, k, "f32") for k, (n, r0) in enumerate(zip(sizes, r0s))]
+    b = mlv.fused.Batch(jobs)
+    print("[headline] DDC mixed FP32 graph batch", flush=True)
+    for _ in range(nsteps // 20):
+        b.graph_step(20)
+    got = [j.state() for j in jobs]
+    b.close()
+    for k, (n, r0) in enumerate(zip(sizes, r0s)):
+        print(f"[headline]   DDC job {k}: {n}^2 R0={r0}", flush=True)
+        alone = _ddc_job(mlv, n, r0, k, "f32")
+        ba = mlv.fused.Batch([alone])
+        ba.step(nsteps)
+        for x, y in zip(alone.state(), got[k]):
+            np.testing.assert_array_equal(x, y)
+        ba.close()
+        d64 = _ddc_job(mlv, n, r0, k, "f64")
+        bd = mlv.fused.Batch([d64])
+        bd.step(nsteps)
+        ref = d64.state()
+        bd.close()
+        errs = [rel(a, r) for a, r in zip(got[k], ref)]
+        print(f"[headline]     FP32 vs FP64 (omega, T, S) {errs}", flush=True)
+        assert max(errs) <= 1e-5
+        if n == 128 and k == 0:
+            from melvin_amd import initial
+
+            p = O.make_params(dict(nx=n, nz=n, lx=DDC_L, lz=DDC_L, initial_dt=1e-3, Pr=7.0,
+                                   R0=r0, tau=0.01))
+            st = O.new_state(p, ("w", "tmp", "xi"), dt=1e-3)
+            st["w"], st["tmp"], st["xi"] = (f.copy() for f in initial.ddc_noise(n, DDC_L, seed=k))
+            for _ in range(nsteps):
+                O.step_ddc(p, st)
+            for a, name in zip(ref, ("w", "tmp", "xi")):
+                assert rel(a, st[name]) <= 1e-12, name

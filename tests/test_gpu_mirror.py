@@ -95,3 +95,28 @@ def test_dump_and_resume_bit_exact(mlv, tmp_path):
     np.testing.assert_array_equal(_h(a[1].gets()), _h(b[1].gets()))
     np.testing.assert_array_equal(_h(a[2].get_all()), _h(b[2].get_all()))
     assert a[0]._t == b[0]._t and a[0]._loop_counter == b[0]._loop_counter
+
+
+@pytest.mark.parametrize("disc", [["spectral", "spectral"], ["spectral", "fdm"]])
+def test_init_var_with_noise_matches_reference_rule(mlv, disc):
+    """utility.init_var_with_noise (reference utility.py:31-39): epsilon *
+    (2U - 1) from default_rng(seed) loaded as physical data, then the
+    truncating forward transform -- the oracle's noise and to_spectral on
+    the same seed, for both discretisations."""
+    from oracle import melvin_oracle as O
+
+    params = dict(nx=64, nz=48, lx=2.0, lz=1.0, final_time=1.0, discretisation=disc)
+    p = mlv.Parameters(params)
+    sim = mlv.Simulation(p, mlv.hip)
+    bfs = [mlv.BasisFunctions.COMPLEX_EXP,
+           mlv.BasisFunctions.FDM if disc[1] == "fdm" else mlv.BasisFunctions.COMPLEX_EXP]
+    v = sim.make_variable("v", bfs)
+    for seed in (0, 11):
+        mlv.utility.init_var_with_noise(v, 1e-3, seed=seed)
+        op = O.make_params(params)
+        want = O.to_spectral(op, O.noise((64, 48), 1e-3, seed))
+        assert rel(_h(v.gets()), want) <= 1e-14
+    # seed 0 is the default, as in the reference
+    mlv.utility.init_var_with_noise(v, 1e-3)
+    assert rel(_h(v.gets()), O.to_spectral(O.make_params(params),
+                                           O.noise((64, 48), 1e-3, 0))) <= 1e-14
