@@ -159,7 +159,13 @@ int mlv_combine(mlv_plan* plan, int moulton, int order, double dt, const void* h
 int mlv_velocity(mlv_plan* plan, const void* w, void* psi, void* ux_spec, void* uz_spec,
                  void* stream);
 /* FP64 reduction of n values of type dtype (b used by SUMSQ2/DOT); writes
- * scale * result to out_f64 (device).  plan may be NULL. */
+ * scale * result to out_f64 (device).  plan may be NULL: the call then uses
+ * its device's shared partials buffer, which the first *_create call on the
+ * device allocates (a plan-less call on a device where nothing was created
+ * yet allocates it).  Reductions that share a partials buffer (the same
+ * plan, or NULL on one device) must not overlap on different streams.
+ * Replaces the reference's xp.max / xp.sum / xp.mean calls
+ * (Integrator.py:35-44, utility.py:42-59). */
 int mlv_reduce(mlv_plan* plan, int op, int dtype, const void* a, const void* b, size_t n,
                double scale, void* out_f64, void* stream);
 

@@ -14,6 +14,8 @@ beside the state (stored separately, so a resumed run is bit-identical).
 copy of the job's arrays is queued on the job's stream (a few microseconds
 of HBM traffic), and the copy to pinned host memory runs on a side stream,
 so the steps queued after the snapshot proceed while the data travels.
+After ``run_device`` (CFL ticker on the device) the job's dt and t are read
+back from the batch first, which synchronises with the device once.
 """
 from __future__ import annotations
 
@@ -42,6 +44,12 @@ def _arrays(job):
 
 
 def _meta(job):
+    owner = getattr(job, "time_owner", None)
+    if owner is not None:
+        # after run_device the CFL ticker may have changed dt on the device:
+        # read the device-side dt and t (a host synchronisation) so that the
+        # checkpoint resumes the run it was taken from
+        owner.sync()
     m = {"format": FORMAT, "dtype": job.dtype, "nx": job.nx, "nz": job.nz, "lx": job.lx,
          "lz": job.lz, "ab_order": job.ab_order, "fd_order": job.fd_order, "alpha": job.alpha,
          "dt": job.dt, "t": job.t, "steps": job.steps, "curr": job.curr}

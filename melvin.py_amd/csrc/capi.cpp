@@ -43,6 +43,21 @@ int ilog2_exact(int n) {
   return l;
 }
 
+// Reduction partials for calls without a plan: one buffer per device.
+// Every plan or batch creation on a device allocates it (see get_twiddles),
+// so a plan-less reduction allocates only when nothing was created on its
+// device before.
+std::mutex g_red_mu;
+std::map<int, double*> g_red;
+
+int global_partials(int device, double** out) {
+  std::lock_guard<std::mutex> lk(g_red_mu);
+  double*& p = g_red[device];
+  if (!p) HIP_TRY(hipMalloc((void**)&p, reduce_partials_bytes()), "hipMalloc partials");
+  *out = p;
+  return MLV_OK;
+}
+
 // FP64-tabulated twiddles exp(-2 pi i k / NMAX), built once per device.
 struct Twiddles {
   void* f32 = nullptr;
@@ -51,10 +66,15 @@ struct Twiddles {
 std::mutex g_tw_mu;
 std::map<int, Twiddles> g_tw;
 
+// Per-device set-up shared by every create call: the twiddle tables and the
+// plan-less reduction partials.
 int get_twiddles(int device, int dtype, const void** out) {
   std::lock_guard<std::mutex> lk(g_tw_mu);
   Twiddles& t = g_tw[device];
   if (!t.f64) {
+    double* part = nullptr;
+    const int rc = global_partials(device, &part);
+    if (rc) return rc;
     // main table exp(-2 pi i k / NMAX), then the per-pass tables of every
     // length (fft.hpp: fft_pass_table), all from long-double angles
     const int total = fft_twiddle_entries();
@@ -104,19 +124,6 @@ size_t max_lds_per_block(int device) {
 }
 
 size_t cplx_bytes(int dtype) { return dtype == MLV_F64 ? 16 : 8; }
-
-// Reduction partials for calls without a plan: one buffer per device,
-// allocated on first use.
-std::mutex g_red_mu;
-std::map<int, double*> g_red;
-
-int global_partials(int device, double** out) {
-  std::lock_guard<std::mutex> lk(g_red_mu);
-  double*& p = g_red[device];
-  if (!p) HIP_TRY(hipMalloc((void**)&p, reduce_partials_bytes()), "hipMalloc partials");
-  *out = p;
-  return MLV_OK;
-}
 
 int copy_tracker(const double* hdr, double* out, int max_records, int* nrec) {
   double h[kDiagHdr];

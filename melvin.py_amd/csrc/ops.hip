@@ -298,8 +298,6 @@ __global__ void __launch_bounds__(kEW) k_flux(const T* __restrict__ ux, const T*
 // FP64 reductions: max, sum, sum(a^2 + b^2), sum(a*b).  Two passes:
 // per-block partials (Neumaier-compensated), then one block combines them.
 // --------------------------------------------------------------------------
-constexpr int kRedBlocks = 1024;
-
 __device__ __forceinline__ double red_combine(double a, double b, int op) {
   return op == MLV_RED_MAX ? fmax(a, b) : a + b;
 }

@@ -31,6 +31,9 @@ hipError_t launch_fd(const void* in, void* out, int rows, int cols, int comp, in
 template <typename T>
 hipError_t launch_flux(const void* ux, const void* uz, const void* f, void* out, int nx, int nz,
                        int order, int xper, int zper, double dx, double dz, hipStream_t s);
+// Blocks of the first reduction stage at most (one (sum, compensation) pair
+// of partials each); the partials buffer holds 2 * kRedBlocks doubles.
+constexpr int kRedBlocks = 1024;
 template <typename T>
 hipError_t launch_reduce(const void* a, const void* b, size_t n, int op, double scale,
                          double* part, double* out, hipStream_t s);

@@ -83,6 +83,8 @@ class FdzJob:
         self.parity = 0
         self.steps = 0
         self.t = 0.0
+        # the batch whose device job table holds dt and t after a run_device
+        self.time_owner = None
 
     @property
     def spectral_shape(self):
@@ -233,6 +235,8 @@ class FdzBatch(fused.DeviceRun):
 
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:
+            if any(getattr(j, "time_owner", None) is self for j in self.jobs):
+                self.sync()  # keep the device-side dt and t of run_device
             _lib.lib().mlv_fdz_batch_destroy(self.handle)
             self.handle = None
 

@@ -66,8 +66,10 @@ class Job:
         self.real = torch.float64 if self.double else torch.float32
         self.cplx = torch.complex128 if self.double else torch.complex64
         self.dt = float(dt)
-        self.coeff = list(coeff) + [0.0] * (4 - len(coeff))
-        self.param = list(param) + [0.0] * (4 - len(param))
+        # plain floats: numpy scalars would not survive the checkpoint's
+        # literal metadata
+        self.coeff = [float(c) for c in coeff] + [0.0] * (4 - len(coeff))
+        self.param = [float(c) for c in param] + [0.0] * (4 - len(param))
         self.ab_order = int(ab_order)
         self.fd_order = int(fd_order)
         self.alpha = float(alpha)
@@ -100,6 +102,9 @@ class Job:
         self.parity = 0
         self.steps = 0
         self.t = 0.0
+        # the batch whose device job table holds this job's dt and t after a
+        # run_device (None when the host values are current)
+        self.time_owner = None
 
     # -- host access --------------------------------------------------------
     @property
@@ -221,6 +226,8 @@ class DeviceRun:
             j.steps += nsteps
             j.curr = (j.curr + nsteps) % j.ab_order
             j.parity = (j.parity + nsteps) & 1
+            # dt and t now live in this batch's device job table
+            j.time_owner = self
 
     def sync(self):
         """Read every job's device-side dt and t; returns the indices of
@@ -232,6 +239,7 @@ class DeviceRun:
             _lib.call(self._C + "_job_time", self.handle, i, ctypes.byref(dt), ctypes.byref(t),
                       ctypes.byref(loops), ctypes.byref(br))
             j.dt, j.t = dt.value, t.value
+            j.time_owner = None
             if br.value:
                 breached.append(i)
         return breached
@@ -342,6 +350,8 @@ class Batch(DeviceRun):
 
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:
+            if any(getattr(j, "time_owner", None) is self for j in self.jobs):
+                self.sync()  # keep the device-side dt and t of run_device
             _lib.lib().mlv_batch_destroy(self.handle)
             self.handle = None
 
