@@ -168,6 +168,18 @@ int mlv_velocity(mlv_plan* plan, const void* w, void* psi, void* ux_spec, void* 
  * (Integrator.py:35-44, utility.py:42-59). */
 int mlv_reduce(mlv_plan* plan, int op, int dtype, const void* a, const void* b, size_t n,
                double scale, void* out_f64, void* stream);
+/* mlv_reduce without a plan that never allocates device memory: no
+ * counterpart in the reference; it is for callers that cannot allocate
+ * (code under HIP graph capture, serving loops and harnesses that forbid
+ * hipMalloc).  Same kernels and results as mlv_reduce, bit for bit.  The
+ * first-stage partials live in a fixed array of the library, 1024
+ * (sum, compensation) pairs = 16 KiB on each device it runs on, held until
+ * the process ends; at most that many partials are used for any n >= 1.
+ * Every call on one device shares that array: calls that overlap on one
+ * device (other streams, other host threads) race, and the caller must
+ * serialise them. */
+int mlv_reduce_noalloc(int op, int dtype, const void* a, const void* b, size_t n, double scale,
+                       void* out_f64, void* stream);
 
 /* ---------------------------------------------------------------------
  * Fused steps over a table of jobs (independent simulations).  One call

@@ -58,6 +58,33 @@ int global_partials(int device, double** out) {
   return MLV_OK;
 }
 
+// First-stage partials of mlv_reduce_noalloc: a file-scope device array of
+// kRedBlocks (sum, compensation) pairs, 16 KiB on each device the library
+// runs on, held until the process ends, so that the call never allocates.
+// launch_reduce uses at most kRedBlocks blocks for any n, so the array fits
+// every input size.  In host code `scratch` only names the symbol: the
+// device address of each device's copy comes from hipGetSymbolAddress (once
+// per device, with that device current), and only that address is passed
+// to kernels.  Every call on a device shares its copy; the partials a
+// reduction reads are all written by its own first stage, so nothing needs
+// clearing between calls.
+static __device__ __align__(256) double scratch[2 * kRedBlocks];
+static_assert(sizeof(scratch) == 2 * kRedBlocks * sizeof(double), "partials size");
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch;
+
+int scratch_partials(int device, double** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch.find(device);
+  if (it == g_scratch.end()) {
+    void* ptr = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    it = g_scratch.emplace(device, static_cast<double*>(ptr)).first;
+  }
+  *out = it->second;
+  return MLV_OK;
+}
+
 // FP64-tabulated twiddles exp(-2 pi i k / NMAX), built once per device.
 struct Twiddles {
   void* f32 = nullptr;
@@ -73,7 +100,8 @@ int get_twiddles(int device, int dtype, const void** out) {
   Twiddles& t = g_tw[device];
   if (!t.f64) {
     double* part = nullptr;
-    const int rc = global_partials(device, &part);
+    int rc = global_partials(device, &part);
+    if (!rc) rc = scratch_partials(device, &part);
     if (rc) return rc;
     // main table exp(-2 pi i k / NMAX), then the per-pass tables of every
     // length (fft.hpp: fft_pass_table), all from long-double angles
@@ -479,6 +507,29 @@ int mlv_reduce(mlv_plan* p, int op, int dtype, const void* a, const void* b, siz
     int rc = global_partials(dev, &part);
     if (rc) return rc;
   }
+  hipError_t e =
+      dtype == MLV_F64
+          ? launch_reduce<double>(a, b, n, op, scale, part, (double*)out, (hipStream_t)stream)
+          : launch_reduce<float>(a, b, n, op, scale, part, (double*)out, (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e, "reduce");
+  return MLV_OK;
+}
+
+int mlv_reduce_noalloc(int op, int dtype, const void* a, const void* b, size_t n, double scale,
+                       void* out, void* stream) {
+  CHECK_PTR(a);
+  CHECK_PTR(out);
+  if (op < MLV_RED_MAX || op > MLV_RED_DOT) return fail(MLV_ERR_ARG, "unknown reduction");
+  if ((op == MLV_RED_SUMSQ2 || op == MLV_RED_DOT) && !b) return fail(MLV_ERR_ARG, "b is NULL");
+  if (dtype != MLV_F32 && dtype != MLV_F64) return fail(MLV_ERR_ARG, "dtype must be F32 or F64");
+  if (n == 0) return fail(MLV_ERR_ARG, "empty reduction");
+  // the first stage runs min(ceil(n / 256), kRedBlocks) blocks, one pair of
+  // partials each: never more than the scratch array holds
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  double* part = nullptr;
+  const int rc = scratch_partials(dev, &part);
+  if (rc) return rc;
   hipError_t e =
       dtype == MLV_F64
           ? launch_reduce<double>(a, b, n, op, scale, part, (double*)out, (hipStream_t)stream)
