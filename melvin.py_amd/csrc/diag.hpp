@@ -6,8 +6,10 @@
 
 namespace mlv {
 
-// partials: max ux, max uz, sum(ux^2 + uz^2), sum(w^2), sum(f1 * uz)
-constexpr int kDiagPart = 5;
+// partials: max ux, max uz, then the Neumaier (sum, compensation) pairs of
+// sum(ux^2 + uz^2), sum(w^2), sum(f1 * uz): the three sums, then the three
+// compensations
+constexpr int kDiagPart = 8;
 // records: loop, t, dt of the step, KE, max ux, max uz, next dt, enstrophy
 // 0.5 mean(w^2), flux mean(f1 * uz) of the first advected scalar (NaN if none)
 constexpr int kDiagRec = 9;

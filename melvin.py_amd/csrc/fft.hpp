@@ -8,8 +8,10 @@
 // 2/4/8 pass for the remainder.  Every thread owns P = 16 or 32 points per
 // pass (P/R butterflies of radix R), so a block of NT threads transforms
 // NT*P/N slots at once.  A pass reads its R inputs into registers,
-// barriers, and writes the R outputs back in place, so one LDS buffer
-// suffices.  Twiddles come from an FP64-tabulated table (rounded once to
+// synchronises, and writes the R outputs back in place, so one LDS buffer
+// suffices.  The threads of a slot stay with it through every pass; when
+// a slot fits one wavefront (N/P <= 64) the synchronisation is the
+// wavefront's own, not a workgroup barrier.  Twiddles come from an FP64-tabulated table (rounded once to
 // float for single precision); the in-register radix-R kernels use exact
 // constants for the trivial angles.
 #pragma once
@@ -107,18 +109,70 @@ __host__ __device__ constexpr int fft_pass_table(int logn, int logns) {
   }
   return off;
 }
-__host__ __device__ constexpr int fft_twiddle_entries() {
+// End of the standard plan's tables.
+__host__ __device__ constexpr int fft_std_entries() {
   int n = NMAX;
   for (int l = 3; l <= LOG_NMAX; ++l) n += fft_tables_size(l);
   return n;
+}
+// Tables for passes outside the standard plan (other orders of the radices),
+// keyed by (logns, logr) alone -- a pass's twiddles depend only on Ns and R:
+// T[r-1][jm] = W_{Ns R}^{jm r}, for 1 <= logr <= 4, 1 <= logns, logns + logr
+// <= LOG_NMAX, placed after the standard tables.  With no match the
+// function returns the end of the keyed tables.
+__host__ __device__ constexpr int fft_keyed_table(int logns, int logr) {
+  int off = fft_std_entries();
+  for (int lr = 1; lr <= 4; ++lr)
+    for (int ln = 1; ln + lr <= LOG_NMAX; ++ln) {
+      if (lr == logr && ln == logns) return off;
+      off += ((1 << lr) - 1) << ln;
+    }
+  return off;
+}
+__host__ __device__ constexpr int fft_twiddle_entries() { return fft_keyed_table(0, 0); }
+
+// Threads that share one transform of N points at P points per thread (0
+// when a thread holds several whole transforms, N < P).  Thread t works on
+// slot t / TPS in every pass, so a slot is touched by TPS consecutive
+// threads only.
+template <int N, int P>
+__host__ __device__ constexpr int fft_tps() {
+  return N >= P ? N / P : 0;
+}
+// Every slot lies inside one wavefront: the passes need no workgroup
+// barrier, only the wavefront's own ordering of its LDS accesses.
+template <int N, int P>
+__host__ __device__ constexpr bool fft_wave_local() {
+  return fft_tps<N, P>() > 0 && fft_tps<N, P>() <= 64;
+}
+
+// Orders the LDS accesses of one wavefront: the compiler may not move
+// memory operations across it, and the LDS executes a wavefront's
+// instructions in issue order, so a read after it sees the wavefront's
+// earlier writes and a write after it cannot overtake earlier reads.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int N, int P>
+__device__ __forceinline__ void fft_sync() {
+  if constexpr (fft_wave_local<N, P>())
+    wave_sync();
+  else
+    __syncthreads();
 }
 
 // One Stockham pass: radix 2^LOGR, sub-transform length Ns = 2^LOGNS.
 // Element i of a slot is at fft_pad(i) = i + i/16.  When the stride between
 // a butterfly's R inputs (or outputs) is a multiple of 16 the padded
 // positions are a fixed stride apart, so every access is one base address
-// plus a compile-time offset.
-template <typename T, int LOGN, int LOGR, int LOGNS, int DIR, int NT, int P>
+// plus a compile-time offset.  With TPS threads per slot, thread t computes
+// butterflies j = t % TPS + it * TPS (it < P/R) of slot t / TPS, so
+// consecutive lanes take consecutive j (consecutive twiddles, conflict-free
+// LDS) and a slot never leaves its threads.
+template <typename T, int LOGN, int LOGR, int LOGNS, int DIR, int NT, int P, bool KEYED = false>
 __device__ __forceinline__ void fft_pass(cx<T>* __restrict__ lds, int active,
                                          const cx<T>* __restrict__ tw) {
   constexpr int N = 1 << LOGN;
@@ -127,18 +181,22 @@ __device__ __forceinline__ void fft_pass(cx<T>* __restrict__ lds, int active,
   constexpr int BF = N / R;          // butterflies per transform
   constexpr int ITEMS = P / R;       // butterflies per thread
   constexpr int LDSN = fft_ldsn(N);
+  constexpr int TPS = fft_tps<N, P>();
   constexpr bool RSEP = (BF % 16) == 0;
   constexpr int RSTR = BF + BF / 16;
   constexpr bool WSEP = (NS % 16) == 0 || (NS == 1 && R == 16);
   constexpr int WSTR = NS == 1 ? 1 : NS + NS / 16;
-  constexpr int TOFF = LOGNS > 0 ? fft_pass_table(LOGN, LOGNS) : 0;
+  constexpr int TOFF =
+      LOGNS == 0 ? 0 : (KEYED ? fft_keyed_table(LOGNS, LOGR) : fft_pass_table(LOGN, LOGNS));
   const int tid = threadIdx.x;
+  // slot and butterfly index of this thread's butterfly `it`
+  auto slot = [&](int it) { return TPS ? tid / TPS : (tid + it * NT) / BF; };
+  auto bfly = [&](int it) { return TPS ? tid % TPS + it * TPS : (tid + it * NT) % BF; };
   cx<T> v[ITEMS][R];
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
-    const int w = tid + it * NT;
-    const int f = w / BF;
-    const int j = w % BF;
+    const int f = slot(it);
+    const int j = bfly(it);
     if (f < active) {
       const cx<T>* src = lds + f * LDSN;
       if constexpr (RSEP) {
@@ -162,12 +220,11 @@ __device__ __forceinline__ void fft_pass(cx<T>* __restrict__ lds, int active,
       Dft<R, DIR, T>::run(v[it]);
     }
   }
-  __syncthreads();
+  fft_sync<N, P>();
 #pragma unroll
   for (int it = 0; it < ITEMS; ++it) {
-    const int w = tid + it * NT;
-    const int f = w / BF;
-    const int j = w % BF;
+    const int f = slot(it);
+    const int j = bfly(it);
     if (f < active) {
       cx<T>* dst = lds + f * LDSN;
       const int jm = j & (NS - 1);
@@ -182,7 +239,7 @@ __device__ __forceinline__ void fft_pass(cx<T>* __restrict__ lds, int active,
       }
     }
   }
-  __syncthreads();
+  fft_sync<N, P>();
 }
 
 template <typename T, int LOGN, int LOGNS, int DIR, int NT, int P>
@@ -198,13 +255,18 @@ struct FftPasses {
 };
 
 // Unnormalised DFT of `active` slots in place.  DIR = -1 forward
-// (exp(-i...)), +1 inverse.  Requires NT*P/N >= active and a barrier
-// between the writes that filled the slots and this call.
-template <typename T, int LOGN, int DIR, int NT, int P>
+// (exp(-i...)), +1 inverse.  Requires NT*P/N >= active and, before the
+// call, a barrier after the writes that filled the slots (a workgroup
+// barrier, or wave_sync() when each wavefront filled its own slots and
+// fft_wave_local holds).  On return the slots are complete for every
+// thread of the block (SYNC_OUT = true) or, with SYNC_OUT = false and
+// fft_wave_local, for the threads of each slot's own wavefront.
+template <typename T, int LOGN, int DIR, int NT, int P, bool SYNC_OUT = true>
 __device__ __forceinline__ void fft_lds(cx<T>* lds, int active, const cx<T>* tw) {
   static_assert(P == 16 || P == 32, "points per thread");
   static_assert((NT * P) % (1 << LOGN) == 0 || (1 << LOGN) > NT * P, "slots");
   FftPasses<T, LOGN, 0, DIR, NT, P>::run(lds, active, tw);
+  if constexpr (SYNC_OUT && fft_wave_local<(1 << LOGN), P>()) __syncthreads();
 }
 
 // Slots available to a block of NT threads at P points per thread.

@@ -27,25 +27,42 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
   const double fx = J.fx, fz = J.fz, fx2 = J.fx2, fz2 = J.fz2;
   const float ffx = (float)fx, ffz = (float)fz, ffx2 = (float)fx2, ffz2 = (float)fz2;
   // Each thread owns ITER = P elements; loads are issued U at a time so
-  // several are in flight before the first is used.
+  // several are in flight before the first is used.  When a column's
+  // transform is private to one wavefront (WL), the thread loads the
+  // elements its own slot's threads transform: column tid / TPS, rows
+  // tid % TPS + i TPS (a column of the kz-major state is contiguous, so the
+  // wavefront reads consecutive addresses) and no workgroup barrier is
+  // needed before the transform.  Otherwise rows are fastest over the
+  // whole block.
   constexpr int ITER = (N * TC) / kNT;
   constexpr int U = 8;
   static_assert(ITER % U == 0, "load batching");
+  constexpr int TPS = fft_tps<N, P>();
+  constexpr bool WL = fft_wave_local<N, P>() && TC * TPS == kNT;
+  auto cx_of = [&](int i, int& c, int& x) {
+    if constexpr (WL) {
+      c = threadIdx.x / TPS;
+      x = threadIdx.x % TPS + i * TPS;
+    } else {
+      const int e = threadIdx.x + i * kNT;
+      c = e >> LOGN;
+      x = e & (N - 1);
+    }
+  };
 #pragma unroll 1
   for (int i0 = 0; i0 < ITER; i0 += U) {
     cx<T> fv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      // rows fastest: a column of the kz-major state is contiguous
-      const int e = threadIdx.x + (i0 + u) * kNT;
-      const int c = e >> LOGN, x = e & (N - 1);
+      int c, x;
+      cx_of(i0 + u, c, x);
       const int r = x <= nn ? x : (x >= N - nn ? x - N + rows : -1);
       fv[u] = (c < ncol && r >= 0) ? src[(size_t)(c0 + c) * rows + r] : czero<T>();
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-    const int e = threadIdx.x + (i0 + u) * kNT;
-    const int c = e >> LOGN, x = e & (N - 1);
+    int c, x;
+    cx_of(i0 + u, c, x);
     cx<T> v = czero<T>();
     if (c < ncol) {
       const int r = x <= nn ? x : (x >= N - nn ? x - N + rows : -1);
@@ -91,7 +108,10 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
     lds[c * LDSN + fft_pad(x)] = v;
     }
   }
-  __syncthreads();
+  if constexpr (WL)
+    wave_sync();
+  else
+    __syncthreads();
   fft_lds<T, LOGN, +1, kNT, P>(lds, ncol, tw);
   // x % kXil fastest, then the column: consecutive lanes store one row
   // group's TC columns, contiguous in the interleaved layout
@@ -242,6 +262,9 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     constexpr int NW = kNT / 64;
     constexpr int SLN = (RB * NIN + 1) / 2;
     constexpr int WPS = SLN >= NW ? 1 : NW / SLN;
+    // slot s is loaded by wavefront s and transformed by the same
+    // wavefront: no workgroup barrier before the inverse transform
+    constexpr bool WL_IN = fft_tps<N, P>() == 64 && WPS == 1 && SL == NW;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int ssub = wv % WPS;
     for (int s = wv / WPS; s < nslot; s += NW / WPS) {
@@ -276,7 +299,10 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
       }
       for (int k = nm + lane + 64 * ssub; k <= N - nm; k += 64 * WPS) z[fft_pad(k)] = czero<T>();
     }
-    __syncthreads();
+    if constexpr (WL_IN)
+      wave_sync();
+    else
+      __syncthreads();
     fft_lds<T, LOGN, +1, kNT, P>(lds, nslot, tw);
     // this job's check loop (J.loops counts the loops before this one)
     const bool diag = (flags & kDiag) && J.loops % (flags >> kDiagShift) == 0;
@@ -341,11 +367,13 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     }
     if (diag) block_diag<kNT>((double)mux, (double)muz, ke, ens, flux, J.diag + (size_t)kDiagPart * local);
     __syncthreads();
-    fft_lds<T, LOGN, -1, kNT, P>(lds, nrows * NEQ, tw);
     // one wavefront per output row (WPO per row when there are fewer rows
-    // than wavefronts)
+    // than wavefronts); when wavefront o transformed slot o itself the
+    // outputs need no workgroup barrier after the transform
     constexpr int NO = RB * NEQ;
     constexpr int WPO = NO >= NW ? 1 : NW / NO;
+    constexpr bool WL_OUT = fft_tps<N, P>() == 64 && NO == NW;
+    fft_lds<T, LOGN, -1, kNT, P, !WL_OUT>(lds, nrows * NEQ, tw);
     const int osub = wv % WPO;
     for (int o = wv / WPO; o < nrows * NEQ; o += NW / WPO) {
       const int r = o / NEQ, eq = o % NEQ;
@@ -525,7 +553,11 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
     }
   }
   __syncthreads();
-  fft_lds<T, LOGN, -1, kNT, P>(lds, ncol, tw);
+  // the epilogue gives column cc to wavefront cc when the block has as many
+  // columns as wavefronts; with 64 threads per transform that wavefront
+  // transformed the column itself
+  constexpr bool WL_OUT = fft_tps<N, P>() == 64 && TC == kNT / 64;
+  fft_lds<T, LOGN, -1, kNT, P, !WL_OUT>(lds, ncol, tw);
 
   const int q = J.q;
   const double dt = J.dt, alpha = J.alpha;

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "fft.hpp"  // wave_sync
+
 namespace mlv {
 
 // Job of logical block b: pref is the exclusive prefix of blocks per job.
@@ -32,14 +34,6 @@ __device__ __forceinline__ int xcd_logical(int nlogical) {
   return l < nlogical ? l : -1;
 }
 static inline unsigned xcd_grid(int n) { return (unsigned)((n + kXcds - 1) / kXcds * kXcds); }
-
-// Ordering of LDS accesses among the lanes of one wavefront (no barrier
-// with the other waves of the block).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 }  // namespace mlv
 
