@@ -118,6 +118,15 @@ int get_twiddles(int device, int dtype, const void** out) {
       f[2 * (size_t)idx + 1] = (float)sn;
     };
     for (int k = 0; k < NMAX; ++k) put(k, k, NMAX);
+    // tables keyed by (Ns, R) (fft_keyed_table)
+    for (int lr = 1; lr <= 4; ++lr)
+      for (int ln = 1; ln + lr <= LOG_NMAX; ++ln) {
+        const int off = fft_keyed_table(ln, lr);
+        const int NS = 1 << ln, R = 1 << lr;
+        for (int r = 1; r < R; ++r)
+          for (int jm = 0; jm < NS; ++jm)
+            put(off + (r - 1) * NS + jm, (long double)((long long)jm * r), (long double)(NS * R));
+      }
     for (int logn = 3; logn <= LOG_NMAX; ++logn) {
       int ns = 0;
       while (ns < logn) {
