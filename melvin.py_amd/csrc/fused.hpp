@@ -307,4 +307,22 @@ hipError_t fused_diag(JobDev<T>* jobs, int njobs, double cutoff, int cadence, hi
 template <typename T>
 hipError_t fused_prepare(const BatchLaunch& bl);
 
+// Launchers and one-time LDS set-up of the three kernel families (one
+// translation unit each: fused_a.hip, fused_b.hip, fused_c.hip).
+template <typename T>
+void launch_xinv(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int step,
+                 const cx<T>* tw, hipStream_t s);
+template <typename T>
+void launch_zpass(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int flags,
+                  const cx<T>* tw, hipStream_t s);
+template <typename T>
+void launch_update(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int step,
+                   const cx<T>* tw, hipStream_t s);
+template <typename T>
+void prepare_xinv(const BatchLaunch& bl);
+template <typename T>
+void prepare_zpass(const BatchLaunch& bl);
+template <typename T>
+void prepare_update(const BatchLaunch& bl);
+
 }  // namespace mlv

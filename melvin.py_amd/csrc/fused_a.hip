@@ -1,0 +1,62 @@
+// Fused step, x-inverse kernels: A (spectral prologue + inverse x pass)
+// and M (single-precision x-mean profiles); the bodies are in fused_xinv.hpp.
+#include "fused_xinv.hpp"
+
+namespace mlv {
+
+template <typename T>
+__global__ void __launch_bounds__(kNT) k_stepA(const JobDev<T>* __restrict__ jobs,
+                                               const int* __restrict__ pref, int njobs, int stage,
+                                               int step, const cx<T>* __restrict__ tw) {
+  extern __shared__ char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int lb = xcd_logical(pref[njobs]);
+  if (lb < 0) return;
+  roleA<T>(jobs, pref, njobs, lb, stage, step, tw, lds);
+}
+
+// Grid: jobs x (kMeanCh / 2) op pairs.
+template <typename T>
+__global__ void __launch_bounds__(kNT) k_stepM(const JobDev<T>* __restrict__ jobs, int stage,
+                                               int step, const cx<T>* __restrict__ tw) {
+  extern __shared__ char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  roleM<T>(jobs[blockIdx.x / (kMeanCh / 2)], blockIdx.x % (kMeanCh / 2), stage, step, tw, lds);
+}
+
+// --------------------------------------------------------------------------
+// Launchers
+// --------------------------------------------------------------------------
+template <typename T>
+void launch_xinv(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int step,
+                 const cx<T>* tw, hipStream_t s) {
+  if (bl.nA[stage] > 0) {
+    MLV_SET_LDS_ONCE(k_stepA<T>, bl.ldsA);
+    hipLaunchKernelGGL(k_stepA<T>, dim3(xcd_grid(bl.nA[stage])), dim3(kNT), bl.ldsA, s, jobs,
+                       (const int*)bl.prefA[stage], bl.njobs, stage, step, tw);
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (bl.ldsM) {
+      MLV_SET_LDS_ONCE(k_stepM<T>, bl.ldsM);
+      hipLaunchKernelGGL(k_stepM<T>, dim3(bl.njobs * (kMeanCh / 2)), dim3(kNT), bl.ldsM, s, jobs,
+                         stage, step, tw);
+    }
+  }
+}
+
+template <typename T>
+void prepare_xinv(const BatchLaunch& bl) {
+  if (bl.ldsA) MLV_SET_LDS_ONCE(k_stepA<T>, bl.ldsA);
+  if constexpr (sizeof(T) == 4) {
+    if (bl.ldsM) MLV_SET_LDS_ONCE(k_stepM<T>, bl.ldsM);
+  }
+}
+
+template void launch_xinv<float>(const JobDev<float>*, const BatchLaunch&, int, int,
+                                 const cx<float>*, hipStream_t);
+template void launch_xinv<double>(const JobDev<double>*, const BatchLaunch&, int, int,
+                                  const cx<double>*, hipStream_t);
+template void prepare_xinv<float>(const BatchLaunch&);
+template void prepare_xinv<double>(const BatchLaunch&);
+
+}  // namespace mlv

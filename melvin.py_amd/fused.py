@@ -338,9 +338,10 @@ class Batch(DeviceRun):
         with each (mlv_batch_autotune); the jobs advance by 2 * steps.
         Returns "wide" or "narrow"."""
         chosen = ctypes.c_int()
+        before = _lib.lib().mlv_batch_steps(self.handle)
         _lib.call("mlv_batch_autotune", self.handle, int(steps), _lib.stream_handle(stream),
                   ctypes.byref(chosen))
-        n = 2 * int(steps)
+        n = _lib.lib().mlv_batch_steps(self.handle) - before
         for j in self.jobs:
             j.steps += n
             j.curr = (j.curr + n) % j.ab_order
