@@ -874,6 +874,8 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
     }
     const bool f64 = dtype == MLV_F64;
     // x passes: narrow (n) and wide (w) points policies, see fused_points
+    const int sla = f64 ? xinv_cols_host<double>(lognx) : xinv_cols_host<float>(lognx);
+    const size_t ldsa = f64 ? xinv_lds_bytes<double>(lognx) : xinv_lds_bytes<float>(lognx);
     const int sln = f64 ? fused_cols_host<double>(lognx, 0) : fused_cols_host<float>(lognx, 0);
     const int slw = f64 ? fused_cols_host<double>(lognx, 1) : fused_cols_host<float>(lognx, 1);
     const size_t ldsn = f64 ? fused_lds_bytes<double>(lognx, 0) : fused_lds_bytes<float>(lognx, 0);
@@ -890,7 +892,8 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
     };
     size_t ldsz = 0;
     for (int s = 0; s < nstage; ++s) ldsz = std::max(ldsz, zlds(bvar[s]));
-    if (ldsn > lds_max || ldsw > lds_max || ldsz > lds_max || sln < 1 || slw < 1) {
+    if (ldsa > lds_max || ldsn > lds_max || ldsw > lds_max || ldsz > lds_max || sla < 1 ||
+        sln < 1 || slw < 1) {
       rc = fail(MLV_ERR_UNSUPPORTED, "job " + std::to_string(j) + ": grid too large for LDS");
       break;
     }
@@ -906,7 +909,7 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
           break;
         }
         const int rb = brows(nin_v, neq_v, slz);
-        a = nA[s] * ((nm + sln - 1) / sln);
+        a = nA[s] * ((nm + sla - 1) / sla);
         c = neq[s] * ((nm + slw - 1) / slw);
         cn = neq[s] * ((nm + sln - 1) / sln);
         bb = (nx + rb - 1) / rb;
@@ -923,7 +926,7 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
       for (int v = 0; v < BV_COUNT; ++v) pB[s][v][j + 1] = pB[s][v][j] + (v == bv ? bb : 0);
     }
     if (rc) break;
-    bl.ldsA = std::max(bl.ldsA, ldsn);
+    bl.ldsA = std::max(bl.ldsA, ldsa);
     // kernel M (single precision mean profiles): one LDS slot of nz
     if (!f64 && jobs[j].mean_split)
       bl.ldsM = std::max(bl.ldsM, (size_t)fft_ldsn(1 << lognz) * 8);

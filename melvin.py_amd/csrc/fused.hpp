@@ -209,6 +209,33 @@ inline size_t fused_lds_bytes(int logn, int wide) {
   return (size_t)s * fft_ldsn(1 << logn) * sizeof(cx<T>);
 }
 
+// Threads per block of the x-inverse kernel A.  Single precision takes
+// 512, so that a block holds 8 columns at 1024 points and each interleaved
+// row pair's store is a whole 128-byte line (two such blocks per CU give as
+// many wavefronts as four blocks of 256); with 64-byte runs the stores of
+// neighbouring tiles only merged in L2 when their blocks ran side by side.
+// Double precision keeps 256 (4 columns of 16-byte values already fill a
+// line, and 512 would not fit the LDS at 4096 points).
+template <typename T>
+__host__ __device__ constexpr int xinv_threads() {
+  return sizeof(T) == 4 ? 512 : 256;
+}
+template <typename T, int LOGN, int P>
+__host__ __device__ constexpr int xinv_cols() {
+  return ((xinv_threads<T>() * P) >> LOGN) < xinv_threads<T>() ? ((xinv_threads<T>() * P) >> LOGN)
+                                                                : xinv_threads<T>();
+}
+template <typename T>
+inline int xinv_cols_host(int logn) {
+  const int s = (xinv_threads<T>() * fused_points<T>(logn, 0)) >> logn;
+  return s < xinv_threads<T>() ? s : xinv_threads<T>();
+}
+template <typename T>
+inline size_t xinv_lds_bytes(int logn) {
+  const int s = (xinv_threads<T>() * fused_points<T>(logn, 0)) >> logn;
+  return (size_t)s * fft_ldsn(1 << logn) * sizeof(cx<T>);
+}
+
 // Rows per z-pass block: as many as fit the LDS slots, at most 8.
 __host__ __device__ constexpr int brows(int nin, int neq, int slots) {
   int best = 1;

@@ -5,7 +5,7 @@
 namespace mlv {
 
 template <typename T>
-__global__ void __launch_bounds__(kNT) k_stepA(const JobDev<T>* __restrict__ jobs,
+__global__ void __launch_bounds__(xinv_threads<T>()) k_stepA(const JobDev<T>* __restrict__ jobs,
                                                const int* __restrict__ pref, int njobs, int stage,
                                                int step, const cx<T>* __restrict__ tw) {
   extern __shared__ char smem_raw[];
@@ -32,7 +32,8 @@ void launch_xinv(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int st
                  const cx<T>* tw, hipStream_t s) {
   if (bl.nA[stage] > 0) {
     MLV_SET_LDS_ONCE(k_stepA<T>, bl.ldsA);
-    hipLaunchKernelGGL(k_stepA<T>, dim3(xcd_grid(bl.nA[stage])), dim3(kNT), bl.ldsA, s, jobs,
+    hipLaunchKernelGGL(k_stepA<T>, dim3(xcd_grid(bl.nA[stage])), dim3(xinv_threads<T>()), bl.ldsA,
+                       s, jobs,
                        (const int*)bl.prefA[stage], bl.njobs, stage, step, tw);
   }
   if constexpr (sizeof(T) == 4) {

@@ -1,7 +1,6 @@
 // Device bodies of the fused step's x-inverse kernels: A (spectral
-// prologue + inverse x pass) and M (single-precision x-mean profiles),
-// shared by their own kernels (fused_a.hip) and the multi-role kernel of
-// the pipelined schedule (fused_multi.hip).
+// prologue + inverse x pass, xinv_threads<T>() threads per block) and M
+// (single-precision x-mean profiles, kNT threads per block).
 #pragma once
 #include "fused.hpp"
 #include "diag_dev.hpp"
@@ -16,9 +15,13 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
                                            int par, const cx<T>* __restrict__ tw, cx<T>* lds) {
   constexpr int N = 1 << LOGN;
   constexpr int P = fused_points<T>(LOGN, 0);
-  constexpr int TC = fused_cols<LOGN, P>();
+  constexpr int NT = xinv_threads<T>();
+  constexpr int TC = xinv_cols<T, LOGN, P>();
   constexpr int LDSN = fft_ldsn(N);
   const int nn = J.nn, nm = J.nm, rows = J.rows;
+  // ops outermost: consecutive blocks write neighbouring tiles of one
+  // channel (measured at 64 x 1024^2: 293 us per launch against 329 us
+  // with a tile's channel ops side by side, although those share its reads)
   const int ntile = (nm + TC - 1) / TC;
   const int opi = local / ntile, tile = local % ntile;
   const XinvOp op = S.a[opi];
@@ -37,17 +40,17 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
   // wavefront reads consecutive addresses) and no workgroup barrier is
   // needed before the transform.  Otherwise rows are fastest over the
   // whole block.
-  constexpr int ITER = (N * TC) / kNT;
+  constexpr int ITER = (N * TC) / NT;
   constexpr int U = 8;
   static_assert(ITER % U == 0, "load batching");
   constexpr int TPS = fft_tps<N, P>();
-  constexpr bool WL = fft_wave_local<N, P>() && TC * TPS == kNT;
+  constexpr bool WL = fft_wave_local<N, P>() && TC * TPS == NT;
   auto cx_of = [&](int i, int& c, int& x) {
     if constexpr (WL) {
       c = threadIdx.x / TPS;
       x = threadIdx.x % TPS + i * TPS;
     } else {
-      const int e = threadIdx.x + i * kNT;
+      const int e = threadIdx.x + i * NT;
       c = e >> LOGN;
       x = e & (N - 1);
     }
@@ -115,12 +118,12 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
     wave_sync();
   else
     __syncthreads();
-  fft_lds<T, LOGN, +1, kNT, P>(lds, ncol, tw);
+  fft_lds<T, LOGN, +1, NT, P>(lds, ncol, tw);
   // x % kXil fastest, then the column: consecutive lanes store one row
   // group's TC columns, contiguous in the interleaved layout
 #pragma unroll 8
   for (int i = 0; i < ITER; ++i) {
-    const int e = threadIdx.x + i * kNT;
+    const int e = threadIdx.x + i * NT;
     const int g = e / kXil;
     const int c = g % TC, x = (g / TC) * kXil + e % kXil;
     if (c < ncol) dst[xrow_base(x, nm) + (size_t)(c0 + c) * kXil] = lds[c * LDSN + fft_pad(x)];
