@@ -26,11 +26,39 @@ struct BShape {
   static constexpr int RB = brows(NIN, NEQ, SL);
   static constexpr int NW = kNT / 64;
   static constexpr int KIT = ((N - 1) / 3 + 63) / 64;  // bound on ceil(nm / 64)
-  static constexpr int SLN = (RB * NIN + 1) / 2;       // input slots
+  // Row-pair slots (an even number of rows): slot s holds channel s % NIN of
+  // the rows x0 + 2 (s / NIN) and x0 + 2 (s / NIN) + 1, which the
+  // interleaved channel layout stores side by side, so one load of two
+  // elements per k fills both halves of the slot and every load instruction
+  // uses whole cache lines.  Otherwise consecutive sequences share a slot.
+  static constexpr bool PAIR = RB % 2 == 0 && kXil == 2;
+  static constexpr int SLN = PAIR ? NIN * (RB / 2) : (RB * NIN + 1) / 2;  // input slots
   static constexpr int WPS = SLN >= NW ? 1 : NW / SLN;  // wavefronts per input slot
+  // slot, and half (0 real, 1 imaginary part), of sequence q = row * NIN +
+  // channel
+  static constexpr int slot_of(int q) { return PAIR ? (q / NIN / 2) * NIN + q % NIN : q / 2; }
+  static constexpr int half_of(int q) { return PAIR ? (q / NIN) % 2 : q % 2; }
+  // slots in use for a block of nrows rows (even when PAIR: RB is even and
+  // the grid sizes are powers of two)
+  __device__ __forceinline__ static int nslots(int nrows) {
+    return PAIR ? NIN * (nrows / 2) : (nrows * NIN + 1) / 2;
+  }
   static constexpr int NO = RB * NEQ;                  // product slots
   static constexpr int WPO = NO >= NW ? 1 : NW / NO;   // wavefronts per product slot
 };
+
+// Two consecutive elements (16 bytes in single precision, one load).
+template <typename T>
+__device__ __forceinline__ void load_two(const cx<T>* __restrict__ p, cx<T>& a, cx<T>& b) {
+  if constexpr (sizeof(T) == 4) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    a = mk<T>(v.x, v.y);
+    b = mk<T>(v.z, v.w);
+  } else {
+    a = p[0];
+    b = p[1];
+  }
+}
 
 // Load phase: the block's input channel rows into the LDS slots, two real
 // sequences per slot packed as one Hermitian spectrum (zero-padded from nm
@@ -46,10 +74,17 @@ __device__ __forceinline__ void bload(const cx<T>* __restrict__ work, size_t chs
   const int ssub = wv % WPS;
   for (int s = wv / WPS; s < nslot; s += NW / WPS) {
     cx<T>* z = lds + s * Sh::LDSN;
-    const int qa = 2 * s, qb = qa + 1;
-    const cx<T>* __restrict__ pa = work + (size_t)(qa % NIN) * chs + xrow_base(x0 + qa / NIN, nm);
-    const cx<T>* __restrict__ pb =
-        qb < nseq ? work + (size_t)(qb % NIN) * chs + xrow_base(x0 + qb / NIN, nm) : nullptr;
+    const cx<T>* __restrict__ pa;
+    const cx<T>* __restrict__ pb = nullptr;
+    if constexpr (Sh::PAIR) {
+      // channel s % NIN, rows x0 + 2 (s / NIN) + {0, 1}: element k of the
+      // pair at 2k and 2k + 1
+      pa = work + (size_t)(s % NIN) * chs + xrow_base(x0 + 2 * (s / NIN), nm);
+    } else {
+      const int qa = 2 * s, qb = qa + 1;
+      pa = work + (size_t)(qa % NIN) * chs + xrow_base(x0 + qa / NIN, nm);
+      if (qb < nseq) pb = work + (size_t)(qb % NIN) * chs + xrow_base(x0 + qb / NIN, nm);
+    }
     // loads of up to KB iterations in flight before any is used
 #pragma unroll 1
     for (int i0 = ssub * KB; i0 < KIT; i0 += WPS * KB) {
@@ -57,8 +92,17 @@ __device__ __forceinline__ void bload(const cx<T>* __restrict__ work, size_t chs
 #pragma unroll
       for (int i = 0; i < KB; ++i) {
         const int k = lane + 64 * (i0 + i);
-        av[i] = k < nm ? pa[(size_t)k * kXil] : czero<T>();
-        bv[i] = (pb && k < nm) ? pb[(size_t)k * kXil] : czero<T>();
+        if constexpr (Sh::PAIR) {
+          if (k < nm) {
+            load_two<T>(pa + (size_t)k * kXil, av[i], bv[i]);
+          } else {
+            av[i] = czero<T>();
+            bv[i] = czero<T>();
+          }
+        } else {
+          av[i] = k < nm ? pa[(size_t)k * kXil] : czero<T>();
+          bv[i] = (pb && k < nm) ? pb[(size_t)k * kXil] : czero<T>();
+        }
       }
 #pragma unroll
       for (int i = 0; i < KB; ++i) {
@@ -179,7 +223,7 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     const int x0 = local * RB;
     const int nrows = min(RB, nx - x0);
     const int nseq = nrows * NIN;
-    const int nslot = (nseq + 1) / 2;
+    const int nslot = Sh::nslots(nrows);
     // slot s is loaded by wavefront s and transformed by the same
     // wavefront: no workgroup barrier before the inverse transform
     constexpr bool WL_IN = fft_tps<N, P>() == 64 && Sh::WPS == 1 && Sh::SL == Sh::NW;
@@ -200,8 +244,8 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
 #pragma unroll
       for (int q = 0; q < RB * NIN; ++q) {
         if (q < nseq) {
-          const cx<T> z = lds[(q >> 1) * LDSN + fft_pad(n)];
-          v[q] = (q & 1) ? z.y : z.x;
+          const cx<T> z = lds[Sh::slot_of(q) * LDSN + fft_pad(n)];
+          v[q] = Sh::half_of(q) ? z.y : z.x;
         } else {
           v[q] = T(0);
         }
@@ -248,7 +292,7 @@ __device__ __forceinline__ void stepB_split(const JobDev<T>& J, const StageDev& 
   const int x0 = local * RB;
   const int nrows = min(RB, nx - x0);
   const int nseq = nrows * NIN;
-  const int nslot = (nseq + 1) / 2;
+  const int nslot = Sh::nslots(nrows);
   constexpr bool WL_IN = fft_tps<N, P>() == 64 && Sh::WPS == 1 && Sh::SL == Sh::NW;
   bload<T, Sh>(J.work, chs, nm, x0, nseq, nslot, lds);
   if constexpr (WL_IN)
@@ -299,7 +343,8 @@ __device__ __forceinline__ void stepB_split(const JobDev<T>& J, const StageDev& 
   for (int r = 0; r < R; ++r) {
     T v[RB * NIN];
 #pragma unroll
-    for (int q = 0; q < RB * NIN; ++q) v[q] = q < nseq ? val[r][q] : T(0);
+    for (int q = 0; q < RB * NIN; ++q)
+      v[q] = q < nseq ? val[r][2 * Sh::slot_of(q) + Sh::half_of(q)] : T(0);
     T px[Sh::NO], pz[Sh::NO];
     bpoint<T, Sh>(J, S, kSplit, diag, flags, x0, nrows, nseq, j + NSL * r, v, d, px, pz);
 #pragma unroll
