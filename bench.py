@@ -178,6 +178,35 @@ def ddc_leg(m, args):
     return res
 
 
+def fdz_roofline(nx, nz, ms_step, kernel_ms, cbytes=8):
+    """HBM roofline of the FD-z step (one Rayleigh-Benard job, FP32 state,
+    FP64 tridiagonal factors).  Algorithmic bytes of one evaluation of the
+    explicit terms, S' = nn nz spectral points (the x transforms stay in
+    LDS, so no physical array is stored), F = 2 fields, AB2: the column
+    kernel reads omega, T and the two spectral velocities and writes the
+    four product spectra (8 S' c); the row kernel reads the four products,
+    the state, one earlier ring slot per field, the residual and T again
+    for the buoyancy (11 S' c), writes the state, the residual, one ring
+    slot per field and the two velocities (8 S' c), and reads the FP64
+    factor tables of its three solves (16 S' each).  The predictor-corrector
+    runs two evaluations per step."""
+    nn = (nx - 1) // 3
+    S = nn * nz
+    cols = 8 * S * cbytes
+    rows = S * cbytes * (11 + 8) + 16 * S * 3
+    step = 2 * (cols + rows)
+    achieved = step / (ms_step * 1e-3) / 1e9
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+           "algorithmic_bytes_per_step": step,
+           "per_kernel": {}}
+    if kernel_ms:
+        for name, b, ms in (("x_transforms", cols, kernel_ms[0]), ("z_rows", rows, kernel_ms[1])):
+            out["per_kernel"][name] = {"bytes_per_launch": b, "ms": ms,
+                                       "GB/s": b / (ms * 1e-3) / 1e9}
+    return out
+
+
 def rbc_leg(m, args):
     """Rayleigh-Benard, Fourier-x / FD-z: nx = 1024 (nn = 341 modes), nz = 513,
     [0,2] x [0,1], Ra = 1e7, Pr = 1, T = 1 - z plus seeded noise 1e-3,
@@ -253,6 +282,7 @@ def rbc_leg(m, args):
         "ms_per_step": r32["seconds"] * 1e3 / steps,
         "fp64_value": nx * nz * steps / r64["seconds"],
         "kernel_ms_per_launch": dict(zip(["x_transforms", "z_rows"], r32["kernel_ms"])),
+        "roofline": fdz_roofline(nx, nz, r32["seconds"] * 1e3 / steps, r32["kernel_ms"]),
         "final_dt": r32["dt"], "final_dt_fp64": r64["dt"],
         "rel_err_fp32_vs_fp64": {"omega": err(r32["state"][0], r64["state"][0]),
                                  "T": err(r32["state"][1], r64["state"][1])},

@@ -575,13 +575,22 @@ __global__ void __launch_bounds__(kNT)
   }
 }
 
-// Thomas factors of one job's operator, one thread per mode (FP64).
+// Thomas factors of one job's operator for mode n, by one wavefront (FP64).
 //   which = 0: Poisson, interior bands [1/dz^2, -(kx^2 + 2/dz^2), 1/dz^2]
 //              (LaplacianSolver.py:21-52)
 //   which > 0: theta diffusion, [-a/dz^2, 1 + a (kx^2 + 2/dz^2), -a/dz^2]
-// First and last rows are identity rows.
-__device__ void fdz_factor_row(double2* __restrict__ fac, int n, int nz, int K, double fxabs,
-                               double dz, int which, double a) {
+// First and last rows (and the padding up to 64 K) are identity rows.  The
+// table is chunk-major (entry i * 64 + lane is element lane * K + i): lane
+// l computes its chunk [l K, l K + K) and stores entry i of it with the
+// other lanes' entries i, so each store instruction writes 64 consecutive
+// entries.  The interior recurrence cp_j = lo / (dg - lo cp_{j-1}) has
+// constant coefficients: a lane runs it from the first row up to its
+// chunk, stopping once it maps a value to itself in floating point (every
+// later factor is then that value; diffusion gets there within a few dozen
+// rows), and then through its chunk -- the same operations in the same
+// order as a single sequential pass, so the table is bit-identical to one.
+__device__ void fdz_factor_wave(double2* __restrict__ fac, int n, int nz, int K, double fxabs,
+                                double dz, int which, double a, int lane) {
   double2* F = fac + (size_t)n * K * 64;
   const double k = (double)n * fxabs;
   const double k2 = k * k;
@@ -594,34 +603,36 @@ __device__ void fdz_factor_row(double2* __restrict__ fac, int n, int nz, int K, 
     lo = -a * idz2;
     dg = 1.0 + a * (k2 + 2.0 / (dz * dz));
   }
-  for (int e = 0; e < K * 64; ++e) F[e] = make_double2(0.0, 1.0);
-  F[0] = make_double2(0.0, 1.0);
-  // The interior recurrence has constant coefficients; once it maps a
-  // value to itself in floating point every later interior factor is that
-  // same value, so the rest of the interior is filled without recomputing
-  // (identical results, far fewer dependent divisions for diffusion).
-  double cprev = 0.0, rprev = 0.0;
-  int j = 1;
-  for (; j < nz - 1; ++j) {
-    const double rd = 1.0 / (dg - lo * cprev);
+  const int j0 = lane * K;
+  // factors of element j - 1 before element j (row 0: cp = 0)
+  double cp = 0.0, rp = 0.0;
+  for (int j = 1; j < j0 && j < nz - 1; ++j) {
+    const double rd = 1.0 / (dg - lo * cp);
     const double c = lo * rd;
-    F[(j % K) * 64 + j / K] = make_double2(c, rd);
-    const bool fixed = c == cprev && rd == rprev;
-    cprev = c;
-    rprev = rd;
-    if (fixed) {
-      for (++j; j < nz - 1; ++j) F[(j % K) * 64 + j / K] = make_double2(c, rd);
-      break;
-    }
+    const bool fixed = c == cp && rd == rp;
+    cp = c;
+    rp = rd;
+    if (fixed) break;
   }
-  // last row: identity (lower 0, diagonal 1), cp = 0
-  F[((nz - 1) % K) * 64 + (nz - 1) / K] = make_double2(0.0, 1.0);
+  for (int i = 0; i < K; ++i) {
+    const int j = j0 + i;
+    double2 v = make_double2(0.0, 1.0);  // identity rows and padding
+    if (j >= 1 && j < nz - 1) {
+      const double rd = 1.0 / (dg - lo * cp);
+      const double c = lo * rd;
+      cp = c;
+      rp = rd;
+      v = make_double2(c, rd);
+    }
+    F[i * 64 + lane] = v;
+  }
 }
 
-__global__ void k_fdz_factors(double2* __restrict__ fac, int nn, int nz, int K, double fxabs,
-                              double dz, int which, double a) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n < nn) fdz_factor_row(fac, n, nz, K, fxabs, dz, which, a);
+// Grid: one 64-thread block per mode.
+__global__ void __launch_bounds__(64) k_fdz_factors(double2* __restrict__ fac, int nn, int nz,
+                                                    int K, double fxabs, double dz, int which,
+                                                    double a) {
+  if ((int)blockIdx.x < nn) fdz_factor_wave(fac, blockIdx.x, nz, K, fxabs, dz, which, a, threadIdx.x);
 }
 
 // One block per job: tracker record and CFL rule (diag_apply) from the
@@ -650,16 +661,16 @@ __global__ void __launch_bounds__(kNT) k_fdz_diag(FdzJob<T>* jobs, double cutoff
 }
 
 // Theta tables of the jobs whose dt the last CFL update changed: grid
-// (modes / 64, jobs, 2 operators).
+// (modes, jobs, 2 operators) of 64-thread blocks.
 template <typename T>
-__global__ void k_fdz_refactor(const FdzJob<T>* __restrict__ jobs) {
+__global__ void __launch_bounds__(64) k_fdz_refactor(const FdzJob<T>* __restrict__ jobs) {
   const FdzJob<T>& J = jobs[blockIdx.y];
   if (!J.refactor || J.scheme == MLV_FDZ_EXPLICIT) return;
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = blockIdx.x;
   if (n >= J.nn) return;
   const int which = 1 + (int)blockIdx.z;
   const double a = J.alpha * J.dt * (which == 1 ? J.Pr : 1.0);
-  fdz_factor_row(J.fac[which], n, J.nz, J.K, J.fx, J.dz, which, a);
+  fdz_factor_wave(J.fac[which], n, J.nz, J.K, J.fx, J.dz, which, a, threadIdx.x);
 }
 
 template <typename T>
@@ -720,8 +731,8 @@ hipError_t fdz_prologue(const FdzJob<T>* jobs, const FdzLaunch& fl, int step, hi
 
 hipError_t fdz_factors(double2* fac, int nn, int nz, int K, double fxabs, double dz, int which,
                        double a, hipStream_t s) {
-  hipLaunchKernelGGL(k_fdz_factors, dim3((nn + 63) / 64), dim3(64), 0, s, fac, nn, nz, K, fxabs,
-                     dz, which, a);
+  hipLaunchKernelGGL(k_fdz_factors, dim3(nn), dim3(64), 0, s, fac, nn, nz, K, fxabs, dz, which,
+                     a);
   return hipGetLastError();
 }
 
@@ -736,7 +747,7 @@ template <typename T>
 hipError_t fdz_diag(FdzJob<T>* jobs, int njobs, int max_nn, double cutoff, int cadence,
                     hipStream_t s) {
   hipLaunchKernelGGL(k_fdz_diag<T>, dim3(njobs), dim3(kNT), 0, s, jobs, cutoff, cadence);
-  hipLaunchKernelGGL(k_fdz_refactor<T>, dim3((max_nn + 63) / 64, njobs, 2), dim3(64), 0, s,
+  hipLaunchKernelGGL(k_fdz_refactor<T>, dim3(max_nn, njobs, 2), dim3(64), 0, s,
                      (const FdzJob<T>*)jobs);
   return hipGetLastError();
 }
