@@ -296,6 +296,28 @@ def rbc_leg(m, args):
         "kinetic_energy": {"fp32": r32["ke"], "fp64": r64["ke"]},
         "nusselt": {"fp32": r32["nu"], "fp64": r64["nu"]},
     }
+    # the same step over a batch of independent runs (distinct noise seeds):
+    # one 1024 x 513 job gives about one wavefront per SIMD per kernel, so
+    # its step is latency-bound; a batch fills the device
+    nb = 8
+    jobs = []
+    for k in range(nb):
+        w, t = initial.rbc_conduction(nx, nz, 2.0, 1.0, seed=k, amp=1e-3)
+        j = fdz.FdzJob(nx, nz, 2.0, 1.0, 1.0, 1e7, dtype="f32", dt=1e-5, scheme="pc")
+        j.set_state([w, t])
+        jobs.append(j)
+    bb = fdz.FdzBatch(jobs)
+    fdz.run_device(bb, 100)
+    torch.cuda.synchronize()
+    t_0 = time.perf_counter()
+    nsteps = 400
+    fdz.run_device(bb, nsteps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t_0
+    bb.close()
+    res["batch_of_8"] = {"jobs": nb, "steps": nsteps, "ms_per_step": el * 1e3 / nsteps,
+                         "value": nb * nx * nz * nsteps / el, "unit": "grid-pt-steps/s",
+                         "roofline_frac": fdz_roofline(nx, nz, el * 1e3 / nsteps / nb, None)["frac"]}
     print(f"[bench] rbc: {res}", file=sys.stderr, flush=True)
     return res
 
@@ -304,10 +326,33 @@ def serving_leg(m, args):
     """Serving: seeded Poisson arrivals at 20 jobs/s for ``serving_seconds``
     (vortex pair or resistive tearing, grids from {256..2048}^2, 100-5000
     steps, FP32), admitted by HBM budget and continuously batched."""
+    import torch
+
     S = m.serving
     reqs = S.poisson_requests(rate=20.0, duration=args.serving_seconds, seed=0)
     srv = S.Server(hbm_budget=96 << 30, chunk=16)
     stats = srv.run(reqs, realtime=True)
+    # cost of one membership change for a resident set like the run's: a
+    # new batch per change (the server's earlier behaviour) against an
+    # in-place rebuild; sets alternate between 64 and 63 jobs
+    pool = [S.make_job(S.Request(i, 0.0, "vortex" if i % 2 else "mhd", (256, 512)[i % 2], 0,
+                                 seed=i)) for i in range(64)]
+    torch.cuda.synchronize()
+    b = m.fused.Batch(pool)
+    t0 = time.perf_counter()
+    for k in range(20):
+        b.close()
+        b = m.fused.Batch(pool[: 64 - (k % 2)])
+    torch.cuda.synchronize()
+    t_new = (time.perf_counter() - t0) / 20
+    t0 = time.perf_counter()
+    for k in range(20):
+        b.rebuild(pool[: 64 - (k % 2)])
+    torch.cuda.synchronize()
+    t_reb = (time.perf_counter() - t0) / 20
+    b.close()
+    stats["membership_change_cost_ms"] = {"new_batch": t_new * 1e3, "rebuild_in_place": t_reb * 1e3,
+                                          "resident_jobs": 64}
     stats.update(arrival_rate_per_s=20.0, arrival_window_s=args.serving_seconds,
                  offered_grid_pt_steps=sum(r.work for r in reqs))
     # FP32 against FP64 for the shortest vortex and MHD requests of the
@@ -358,7 +403,7 @@ def main():
     ap.add_argument("--ddc-steps", type=int, default=500)
     ap.add_argument("--rbc-steps", type=int, default=2000,
                     help="steps of the Rayleigh-Benard FD-z leg (0 skips it)")
-    ap.add_argument("--serving-seconds", type=float, default=8.0,
+    ap.add_argument("--serving-seconds", type=float, default=60.0,
                     help="serving leg: length of the Poisson arrival window (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (a known-size kernel "
@@ -444,8 +489,13 @@ def main():
     b64 = fused.Batch([job64])
     el64, kms64, _ = timed(b64, steps64, args.warmup)
     value64 = NX * NZ * steps64 / el64
+    # compare after the same number of steps (warm-up, tuning, timed and
+    # kernel-timing steps may differ between the two runs)
+    n32 = jobs32[0].steps
+    if job64.steps < n32:
+        b64.step(n32 - job64.steps)
     w64 = job64.state()[0]
-    err = float(np.linalg.norm(w32 - w64) / np.linalg.norm(w64)) if steps64 == args.steps else None
+    err = float(np.linalg.norm(w32 - w64) / np.linalg.norm(w64)) if job64.steps == n32 else None
     print(f"[bench] fp64 plain: {el64 * 1e3 / steps64:.3f} ms/step, {value64:.3e} "
           f"grid-pt-steps/s; fp32 vs fp64 omega rel L2 {err}", file=sys.stderr, flush=True)
 
@@ -462,7 +512,7 @@ def main():
              "kernel_ms_per_launch": dict(zip(["x_inverse", "z_pass", "x_forward_update"], kms_s)),
              "rel_err_fp32_vs_fp64_omega":
                  float(np.linalg.norm(w_s - w64) / np.linalg.norm(w64))
-                 if steps64 == args.steps else None}
+                 if jobs_s[0].steps == job64.steps else None}
     print(f"[bench] fp32 batched, mean split: {split['ms_per_step']:.3f} ms/step, "
           f"{split['value']:.3e} grid-pt-steps/s; omega rel L2 {split['rel_err_fp32_vs_fp64_omega']}",
           file=sys.stderr, flush=True)

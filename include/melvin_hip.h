@@ -226,6 +226,17 @@ typedef struct mlv_job_desc {
 int mlv_job_work_bytes(int problem, int nfield, int nx, int nz, int dtype, size_t* bytes);
 int mlv_batch_create(mlv_batch** batch, int dtype, const mlv_job_desc* jobs, int njobs);
 int mlv_batch_destroy(mlv_batch* batch);
+/* Replace the batch's job set (jobs leave, new jobs join, the others keep
+ * going): the same tables mlv_batch_create builds, written into the
+ * batch's device memory, which grows only when the new set needs more.
+ * Each job that was already in the batch (same workspace pointer) keeps its
+ * tracker records; the jobs' clocks, ring slots and dt come from the
+ * descriptors as at creation, and the batch's step count restarts at 0.
+ * Synchronises the device first (queued steps may still read the old
+ * tables) and drops a captured graph.  On an invalid descriptor the batch
+ * is left unchanged.  Serves continuous batching (the reference has no
+ * batches). */
+int mlv_batch_rebuild(mlv_batch* batch, const mlv_job_desc* jobs, int njobs);
 /* Run nsteps steps; write_phys = 1 also writes the physical fields of the
  * last of them.  Each step reads dt from the job table. */
 int mlv_batch_step(mlv_batch* batch, int nsteps, int write_phys, void* stream);

@@ -34,7 +34,7 @@ EXPORTS = (
     "mlv_last_error", "mlv_abi_version", "mlv_plan_create", "mlv_plan_destroy",
     "mlv_plan_info", "mlv_to_physical", "mlv_to_spectral", "mlv_spec_op",
     "mlv_fd_op", "mlv_flux_divergence", "mlv_poisson", "mlv_update",
-    "mlv_reduce", "mlv_reduce_noalloc", "mlv_combine", "mlv_velocity", "mlv_job_work_bytes", "mlv_batch_create",
+    "mlv_reduce", "mlv_reduce_noalloc", "mlv_combine", "mlv_velocity", "mlv_job_work_bytes", "mlv_batch_create", "mlv_batch_rebuild",
     "mlv_batch_destroy", "mlv_batch_step", "mlv_batch_graph_step", "mlv_batch_set_dt",
     "mlv_batch_steps", "mlv_batch_profile", "mlv_batch_profile_read",
     "mlv_batch_run", "mlv_batch_job_time", "mlv_batch_tracker", "mlv_batch_autotune",
@@ -135,6 +135,7 @@ def _declare(lib):
         "mlv_velocity": ([vp, vp, vp, vp, vp, vp], i),
         "mlv_job_work_bytes": ([i, i, i, i, i, ctypes.POINTER(sz)], i),
         "mlv_batch_create": ([pp, i, ctypes.POINTER(JobDesc), i], i),
+        "mlv_batch_rebuild": ([vp, ctypes.POINTER(JobDesc), i], i),
         "mlv_batch_destroy": ([vp], i),
         "mlv_batch_step": ([vp, i, i, vp], i),
         "mlv_batch_graph_step": ([vp, i, vp], i),

@@ -601,7 +601,6 @@ struct BatchHost {
   BatchLaunch bl{};
   long long steps = 0;
   const void* tw = nullptr;
-  std::vector<void*> allocs;
   // optional kernel timing: 4 events per stage and step (before A, B, C, after C)
   // captured sequence of graph_steps steps (see mlv_batch_graph_step)
   hipStream_t cap_stream = nullptr;
@@ -616,10 +615,20 @@ struct BatchHost {
   double ms[3] = {0, 0, 0};
   long long launches[3] = {0, 0, 0};
   std::vector<char> host;          // host copy of the job table
-  std::vector<double*> diag;       // per job diagnostics buffer
+  std::vector<double*> diag;       // per job diagnostics buffer (in diag_buf)
   std::vector<int> nbB;
   std::vector<long long> loops0;   // per job loop count at creation
   std::string signature;           // the job set's shapes (batch_signature)
+  // device memory: the job table and every prefix array in one allocation,
+  // the jobs' diagnostics in one of two buffers (a rebuild copies the
+  // records of the jobs that stay into the other); each is reused while a
+  // rebuilt job set fits it
+  void* tables = nullptr;
+  size_t tables_cap = 0;
+  double* diag_buf[2] = {nullptr, nullptr};
+  size_t diag_cap[2] = {0, 0};
+  int diag_cur = 0;
+  std::vector<const void*> keys;   // a job's identity across rebuilds: its workspace
 };
 
 // Kernel C tile shapes picked by mlv_batch_autotune, by job-set signature;
@@ -813,31 +822,37 @@ int mlv_job_work_bytes(int problem, int nfield, int nx, int nz, int dtype, size_
   return MLV_OK;
 }
 
-int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int njobs) {
-  if (!out || !jobs || njobs <= 0) return fail(MLV_ERR_ARG, "bad batch arguments");
-  *out = nullptr;
-  if (dtype != MLV_F32 && dtype != MLV_F64) return fail(MLV_ERR_ARG, "dtype must be F32 or F64");
-  BatchHost* b = new BatchHost();
-  b->dtype = dtype;
-  b->njobs = njobs;
-  HIP_TRY(hipGetDevice(&b->device), "hipGetDevice");
-  int rc = get_twiddles(b->device, dtype, &b->tw);
-  if (rc) {
-    delete b;
-    return rc;
-  }
-  const size_t lds_max = max_lds_per_block(b->device);
-  const size_t jsz = dtype == MLV_F64 ? sizeof(JobDev<double>) : sizeof(JobDev<float>);
-  std::vector<char> host(jsz * njobs);
+// Host plan of a batch: the device job table and every launch table of
+// the job set, ready for one upload (mlv_batch_create, mlv_batch_rebuild).
+struct BatchPlan {
+  std::vector<char> host;  // JobDev table
   std::vector<int> pA[2], pB[2][BV_COUNT], pC[2], pCn[2];
+  BatchLaunch bl{};        // block counts and LDS sizes; pointers set by upload_batch
   size_t ldsCn = 0;
+  std::vector<int> nbB;
+  std::vector<long long> loops0;
+  std::string signature;
+  int nstage = 1;
+};
+
+static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device, BatchPlan& P) {
+  int rc = MLV_OK;
+  const size_t lds_max = max_lds_per_block(device);
+  const size_t jsz = dtype == MLV_F64 ? sizeof(JobDev<double>) : sizeof(JobDev<float>);
+  std::vector<char>& host = P.host;
+  host.assign(jsz * njobs, 0);
+  auto& pA = P.pA;
+  auto& pB = P.pB;
+  auto& pC = P.pC;
+  auto& pCn = P.pCn;
+  size_t& ldsCn = P.ldsCn;
   for (int s = 0; s < 2; ++s) {
     pA[s].assign(njobs + 1, 0);
     pC[s].assign(njobs + 1, 0);
     pCn[s].assign(njobs + 1, 0);
     for (int v = 0; v < BV_COUNT; ++v) pB[s][v].assign(njobs + 1, 0);
   }
-  BatchLaunch& bl = b->bl;
+  BatchLaunch& bl = P.bl;
   std::memset(&bl, 0, sizeof(bl));
   bl.njobs = njobs;
   for (int j = 0; j < njobs; ++j) {
@@ -897,7 +912,7 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
       rc = fail(MLV_ERR_UNSUPPORTED, "job " + std::to_string(j) + ": grid too large for LDS");
       break;
     }
-    if (nstage > b->nstage) b->nstage = nstage;
+    if (nstage > P.nstage) P.nstage = nstage;
     for (int s = 0; s < 2; ++s) {
       int a = 0, c = 0, cn = 0, bb = 0, bv = 0;
       if (s < nstage) {
@@ -917,8 +932,8 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
         bl.ldsB[bv] = std::max(bl.ldsB[bv], zlds(bv));
       }
       if (s == 0) {
-        b->nbB.push_back(bb);
-        b->loops0.push_back(jobs[j].loops);
+        P.nbB.push_back(bb);
+        P.loops0.push_back(jobs[j].loops);
       }
       pA[s][j + 1] = pA[s][j] + a;
       pC[s][j + 1] = pC[s][j] + c;
@@ -942,79 +957,149 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
     // register file either way and runs wide when its grid covers the
     // device's block slots twice over at two blocks per CU.
     int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess ||
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
         cus <= 0)
       cus = 256;
-    bl.wideC = (b->dtype == MLV_F64 && pC[0][njobs] >= 4 * cus) ? 1 : 0;
+    bl.wideC = (dtype == MLV_F64 && pC[0][njobs] >= 4 * cus) ? 1 : 0;
     // a shape picked by mlv_batch_autotune for the same set of jobs wins
-    b->signature = batch_signature(dtype, jobs, njobs);
+    P.signature = batch_signature(dtype, jobs, njobs);
     {
       std::lock_guard<std::mutex> lk(g_tuned_mu);
-      auto it = g_tuned.find(b->signature);
+      auto it = g_tuned.find(P.signature);
       if (it != g_tuned.end()) bl.wideC = it->second;
     }
     bl.ldsCp[1] = bl.ldsC;
     bl.ldsCp[0] = ldsCn;
   }
-  if (rc) {
-    delete b;
-    return rc;
+  return rc;
+}
+
+// The tables of plan P on the device: the job table (with each job's
+// diagnostics buffer) and the prefix arrays in one allocation, grown only
+// when P needs more; the diagnostics buffers in the other of the two
+// diagnostics allocations, with the records of jobs that were in the batch
+// before (matched by workspace pointer) copied over and the others
+// cleared.  The device must be idle (the old tables may still be read by
+// queued kernels otherwise).
+static int upload_batch(BatchHost* b, BatchPlan& P, const mlv_job_desc* jobs, int njobs) {
+  const size_t jsz = b->dtype == MLV_F64 ? sizeof(JobDev<double>) : sizeof(JobDev<float>);
+  auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  // layout of the tables allocation
+  size_t off = align(P.host.size());
+  std::vector<std::pair<const std::vector<int>*, int**>> arrays;
+  BatchLaunch& bl = P.bl;
+  for (int s = 0; s < 2; ++s) {
+    arrays.push_back({&P.pA[s], &bl.prefA[s]});
+    arrays.push_back({&P.pC[s], &bl.prefCp[1][s]});
+    arrays.push_back({&P.pCn[s], &bl.prefCp[0][s]});
+    for (int v = 0; v < BV_COUNT; ++v) arrays.push_back({&P.pB[s][v], &bl.prefB[s][v]});
   }
-  auto dalloc = [&](size_t bytes, void** p) -> int {
-    hipError_t e = hipMalloc(p, bytes);
-    if (e != hipSuccess) return hip_fail(e, "hipMalloc batch");
-    b->allocs.push_back(*p);
-    return MLV_OK;
-  };
-  auto upload_pref = [&](const std::vector<int>& v, int** dst) -> int {
-    int r = dalloc(v.size() * sizeof(int), (void**)dst);
-    if (r) return r;
-    HIP_TRY(hipMemcpy(*dst, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice), "upload");
-    return MLV_OK;
-  };
-  rc = dalloc(host.size(), &b->jobs_dev);
-  for (int s = 0; s < 2 && !rc; ++s) {
-    bl.nA[s] = pA[s][njobs];
-    bl.nCp[1][s] = pC[s][njobs];
-    bl.nCp[0][s] = pCn[s][njobs];
-    rc = upload_pref(pA[s], &bl.prefA[s]);
-    if (!rc) rc = upload_pref(pC[s], &bl.prefCp[1][s]);
-    if (!rc) rc = upload_pref(pCn[s], &bl.prefCp[0][s]);
-    for (int v = 0; v < BV_COUNT && !rc; ++v) {
-      bl.nB[s][v] = pB[s][v][njobs];
-      rc = upload_pref(pB[s][v], &bl.prefB[s][v]);
-    }
+  std::vector<size_t> aoff;
+  for (auto& a : arrays) {
+    aoff.push_back(off);
+    off = align(off + a.first->size() * sizeof(int));
   }
-  if (!rc) select_c_policy(bl, bl.wideC);
-  for (int j = 0; j < njobs && !rc; ++j) {
-    double* dg = nullptr;
-    const size_t bytes = diag_doubles(b->nbB[j]) * sizeof(double);
-    rc = dalloc(bytes, (void**)&dg);
-    if (!rc) {
-      hipError_t e = hipMemset(dg, 0, bytes);
-      if (e != hipSuccess) rc = hip_fail(e, "clear diagnostics");
-    }
-    if (rc) break;
-    b->diag.push_back(dg);
-    if (dtype == MLV_F64) {
-      reinterpret_cast<JobDev<double>*>(host.data())[j].diag = dg;
-      reinterpret_cast<JobDev<double>*>(host.data())[j].nbB = b->nbB[j];
+  const size_t tbytes = off;
+  if (tbytes > b->tables_cap) {
+    if (b->tables) (void)hipFree(b->tables);
+    b->tables = nullptr;
+    b->tables_cap = 0;
+    HIP_TRY(hipMalloc(&b->tables, tbytes), "hipMalloc batch tables");
+    b->tables_cap = tbytes;
+  }
+  char* base = static_cast<char*>(b->tables);
+  for (size_t k = 0; k < arrays.size(); ++k)
+    *arrays[k].second = reinterpret_cast<int*>(base + aoff[k]);
+  for (int s = 0; s < 2; ++s) {
+    bl.nA[s] = P.pA[s][njobs];
+    bl.nCp[1][s] = P.pC[s][njobs];
+    bl.nCp[0][s] = P.pCn[s][njobs];
+    for (int v = 0; v < BV_COUNT; ++v) bl.nB[s][v] = P.pB[s][v][njobs];
+  }
+  // diagnostics: the buffer not in use now, grown when needed
+  std::vector<size_t> doff(njobs);
+  size_t dtotal = 0;
+  for (int j = 0; j < njobs; ++j) {
+    doff[j] = dtotal;
+    dtotal += align(diag_doubles(P.nbB[j]) * sizeof(double));
+  }
+  const int nd = b->diag_buf[b->diag_cur] ? 1 - b->diag_cur : b->diag_cur;
+  if (dtotal > b->diag_cap[nd]) {
+    if (b->diag_buf[nd]) (void)hipFree(b->diag_buf[nd]);
+    b->diag_buf[nd] = nullptr;
+    b->diag_cap[nd] = 0;
+    HIP_TRY(hipMalloc((void**)&b->diag_buf[nd], dtotal), "hipMalloc diagnostics");
+    b->diag_cap[nd] = dtotal;
+  }
+  std::map<const void*, int> old;
+  for (size_t j = 0; j < b->keys.size(); ++j) old[b->keys[j]] = (int)j;
+  std::vector<double*> diag(njobs);
+  std::vector<const void*> keys(njobs);
+  for (int j = 0; j < njobs; ++j) {
+    diag[j] = reinterpret_cast<double*>(reinterpret_cast<char*>(b->diag_buf[nd]) + doff[j]);
+    keys[j] = jobs[j].work;
+    const size_t bytes = diag_doubles(P.nbB[j]) * sizeof(double);
+    auto it = old.find(keys[j]);
+    if (it != old.end() && b->nbB[it->second] == P.nbB[j]) {
+      HIP_TRY(hipMemcpy(diag[j], b->diag[it->second], bytes, hipMemcpyDeviceToDevice),
+              "carry diagnostics");
     } else {
-      reinterpret_cast<JobDev<float>*>(host.data())[j].diag = dg;
-      reinterpret_cast<JobDev<float>*>(host.data())[j].nbB = b->nbB[j];
+      HIP_TRY(hipMemset(diag[j], 0, bytes), "clear diagnostics");
+    }
+    if (b->dtype == MLV_F64) {
+      reinterpret_cast<JobDev<double>*>(P.host.data())[j].diag = diag[j];
+      reinterpret_cast<JobDev<double>*>(P.host.data())[j].nbB = P.nbB[j];
+    } else {
+      reinterpret_cast<JobDev<float>*>(P.host.data())[j].diag = diag[j];
+      reinterpret_cast<JobDev<float>*>(P.host.data())[j].nbB = P.nbB[j];
     }
   }
-  if (!rc) {
-    hipError_t e = hipMemcpy(b->jobs_dev, host.data(), host.size(), hipMemcpyHostToDevice);
-    if (e != hipSuccess) rc = hip_fail(e, "upload jobs");
-    b->host = host;
-  }
-  if (!rc) {
-    hipError_t e = dtype == MLV_F64 ? fused_prepare<double>(bl) : fused_prepare<float>(bl);
-    if (e != hipSuccess) rc = hip_fail(e, "kernel attributes");
-  }
+  // one upload of the job table and the prefix arrays
+  std::vector<char> stage(tbytes, 0);
+  std::memcpy(stage.data(), P.host.data(), P.host.size());
+  for (size_t k = 0; k < arrays.size(); ++k)
+    std::memcpy(stage.data() + aoff[k], arrays[k].first->data(), arrays[k].first->size() * sizeof(int));
+  HIP_TRY(hipMemcpy(b->tables, stage.data(), tbytes, hipMemcpyHostToDevice), "upload batch tables");
+  (void)jsz;
+  b->diag_cur = nd;
+  b->diag = diag;
+  b->keys = keys;
+  b->jobs_dev = b->tables;
+  b->njobs = njobs;
+  b->nstage = P.nstage;
+  b->nbB = P.nbB;
+  b->loops0 = P.loops0;
+  b->signature = P.signature;
+  b->host = P.host;
+  b->steps = 0;
+  b->bl = bl;
+  select_c_policy(b->bl, b->bl.wideC);
+  hipError_t e = b->dtype == MLV_F64 ? fused_prepare<double>(b->bl) : fused_prepare<float>(b->bl);
+  if (e != hipSuccess) return hip_fail(e, "kernel attributes");
+  return MLV_OK;
+}
+
+static void free_batch_memory(BatchHost* b) {
+  if (b->tables) (void)hipFree(b->tables);
+  for (int k = 0; k < 2; ++k)
+    if (b->diag_buf[k]) (void)hipFree(b->diag_buf[k]);
+  b->tables = nullptr;
+  b->diag_buf[0] = b->diag_buf[1] = nullptr;
+}
+
+int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int njobs) {
+  if (!out || !jobs || njobs <= 0) return fail(MLV_ERR_ARG, "bad batch arguments");
+  *out = nullptr;
+  if (dtype != MLV_F32 && dtype != MLV_F64) return fail(MLV_ERR_ARG, "dtype must be F32 or F64");
+  BatchHost* b = new BatchHost();
+  b->dtype = dtype;
+  HIP_TRY(hipGetDevice(&b->device), "hipGetDevice");
+  int rc = get_twiddles(b->device, dtype, &b->tw);
+  BatchPlan P;
+  if (!rc) rc = plan_batch(dtype, jobs, njobs, b->device, P);
+  if (!rc) rc = upload_batch(b, P, jobs, njobs);
   if (rc) {
-    for (void* p : b->allocs) (void)hipFree(p);
+    free_batch_memory(b);
     delete b;
     return rc;
   }
@@ -1022,11 +1107,28 @@ int mlv_batch_create(mlv_batch** out, int dtype, const mlv_job_desc* jobs, int n
   return MLV_OK;
 }
 
+int mlv_batch_rebuild(mlv_batch* bp, const mlv_job_desc* jobs, int njobs) {
+  if (!bp || !jobs || njobs <= 0) return fail(MLV_ERR_ARG, "bad batch arguments");
+  BatchHost* b = reinterpret_cast<BatchHost*>(bp);
+  BatchPlan P;
+  int rc = plan_batch(b->dtype, jobs, njobs, b->device, P);
+  if (rc) return rc;  // the batch is unchanged
+  // queued steps may still read the tables about to be overwritten
+  HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  int rc2 = prof_collect(b);
+  if (rc2) return rc2;
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+  return upload_batch(b, P, jobs, njobs);
+}
+
 int mlv_batch_destroy(mlv_batch* bp) {
   if (!bp) return MLV_OK;
   BatchHost* b = reinterpret_cast<BatchHost*>(bp);
   (void)hipDeviceSynchronize();
-  for (void* p : b->allocs) (void)hipFree(p);
+  free_batch_memory(b);
   for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
   if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);

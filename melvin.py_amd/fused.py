@@ -301,6 +301,23 @@ class Batch(DeviceRun):
         self.handle = h
         self._steps0 = [j.steps for j in self.jobs]
 
+    def rebuild(self, jobs):
+        """Change the batch's membership in place (mlv_batch_rebuild): jobs
+        not in ``jobs`` leave, new ones join, jobs that stay keep their
+        tracker records; no memory is allocated while the new set fits the
+        batch's tables.  Synchronises with the device."""
+        jobs = list(jobs)
+        if not jobs:
+            raise ValueError("empty batch")
+        if {j.dtype for j in jobs} != {self.jobs[0].dtype}:
+            raise ValueError("all jobs of a batch must share one precision")
+        if any(getattr(j, "time_owner", None) is self for j in self.jobs):
+            self.sync()  # the device-side dt and t of run_device
+        arr = (_lib.JobDesc * len(jobs))(*[j.desc() for j in jobs])
+        _lib.call("mlv_batch_rebuild", self.handle, arr, len(jobs))
+        self.jobs = jobs
+        self._steps0 = [j.steps for j in self.jobs]
+
     def step(self, nsteps=1, write_phys=False, stream=None):
         """Queue ``nsteps`` steps of every job on the stream (asynchronous)."""
         if nsteps <= 0:

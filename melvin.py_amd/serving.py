@@ -10,8 +10,10 @@ others run overlaps on the device.  Each job's arithmetic is independent of
 what else is in the batch, so a result is bit-identical alone and in a mix.
 
 Reported per request: admission wait and completion latency (completion
-time minus arrival time); overall: aggregate grid-point-steps/s and the
-p50 / p99 latency.
+time minus arrival time); overall: aggregate grid-point-steps/s, the
+p50 / p99 latency and admission wait, and the time spent changing the
+batch's membership (in place with ``Batch.rebuild``, or by creating a new
+batch each time with ``incremental=False``).
 """
 from __future__ import annotations
 
@@ -110,7 +112,8 @@ class Server:
     """
 
     def __init__(self, hbm_budget=64 << 30, chunk=16, dtype="f32", keep_results=False,
-                 job_factory=make_job, virtual_step_s=1e-3):
+                 job_factory=make_job, virtual_step_s=1e-3, incremental=True):
+        self.incremental = bool(incremental)
         self.budget = int(hbm_budget)
         self.chunk = int(chunk)
         self.dtype = dtype
@@ -132,6 +135,8 @@ class Server:
         total_work = 0
         chunks = 0
         max_res = 0
+        rebuild_s = 0.0
+        rebuilds = 0
         while pending or waiting or resident:
             now = clock() if realtime else sim_t
             while pending and pending[0].arrival <= now:
@@ -158,9 +163,16 @@ class Server:
                         sim_t = nxt
                 continue
             if dirty:
-                if batch is not None:
-                    batch.close()
-                batch = fused.Batch([e[1] for e in resident])
+                tb = time.perf_counter()
+                members = [e[1] for e in resident]
+                if batch is not None and self.incremental:
+                    batch.rebuild(members)
+                else:
+                    if batch is not None:
+                        batch.close()
+                    batch = fused.Batch(members)
+                rebuild_s += time.perf_counter() - tb
+                rebuilds += 1
                 dirty = False
             k = min(self.chunk, min(e[2] for e in resident))
             batch.step(k)
@@ -192,6 +204,7 @@ class Server:
         if batch is not None:
             batch.close()
         lat = np.array([r.latency for r in reqs if r.latency is not None])
+        wait = np.array([r.admitted - r.arrival for r in reqs if r.admitted is not None])
         return {
             "requests": len(reqs),
             "wall_s": wall,
@@ -200,6 +213,11 @@ class Server:
             "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
             "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
             "mean_latency_s": float(lat.mean()) if len(lat) else None,
+            "p50_admission_wait_s": float(np.percentile(wait, 50)) if len(wait) else None,
+            "p99_admission_wait_s": float(np.percentile(wait, 99)) if len(wait) else None,
             "chunks": chunks,
             "max_resident": max_res,
+            "membership_changes": rebuilds,
+            "membership_change_s": rebuild_s,
+            "membership_change_mode": "rebuild in place" if self.incremental else "new batch",
         }
