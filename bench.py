@@ -446,9 +446,9 @@ def main():
         return jobs
 
     def timed(batch, steps, warmup):
-        """The warm-up steps the batch and then picks the update kernel's
-        tile shape by measurement (Batch.autotune: real steps with each
-        shape), ``warmup`` steps in all; then the wall time of ``steps`` steps (no
+        """The warm-up steps the batch and then picks each fused kernel's
+        variant by measurement (Batch.autotune: real steps with each
+        option), ``warmup`` steps in all; then the wall time of ``steps`` steps (no
         per-kernel events inside the timed region), then the average
         duration of each kernel over a separate run of up to 50 steps timed
         with HIP events on the launch stream."""
@@ -471,7 +471,7 @@ def main():
         m._lib.call("mlv_batch_profile", batch.handle, 0)
         return el, [ms[k] / max(n[k], 1) for k in range(3)], [n[k] for k in range(3)]
 
-    shapes = []  # update kernel tile shape picked for each leg
+    shapes = []  # kernel choices picked by the autotuner for each leg
 
     # batched FP32 (the headline)
     jobs32 = make_jobs(args.batch, "f32")
@@ -508,7 +508,7 @@ def main():
     w_s = jobs_s[0].state()[0]
     b_s.close()
     split = {"value": pts * args.steps / el_s, "ms_per_step": el_s * 1e3 / args.steps,
-             "update_kernel_shape": shapes[2],
+             "kernel_plan": shapes[2],
              "kernel_ms_per_launch": dict(zip(["x_inverse", "z_pass", "x_forward_update"], kms_s)),
              "rel_err_fp32_vs_fp64_omega":
                  float(np.linalg.norm(w_s - w64) / np.linalg.norm(w64))
@@ -550,7 +550,7 @@ def main():
         "ratio_batched_fp32_vs_plain_fp64": value / value64,
         "rel_err_fp32_vs_fp64_omega": err,
         "kernel_ms_per_launch": dict(zip(names, kms32)),
-        "update_kernel_shape": {"fp32_batched": shapes[0], "fp64_plain": shapes[1]},
+        "kernel_plan": {"fp32_batched": shapes[0], "fp64_plain": shapes[1]},
         "mean_split": False,
         "fp32_batched_with_mean_split": split,
         "roofline": {"bound": "hbm", "kernel": names[dom], "achieved": achieved,

@@ -38,6 +38,7 @@ EXPORTS = (
     "mlv_batch_destroy", "mlv_batch_step", "mlv_batch_graph_step", "mlv_batch_set_dt",
     "mlv_batch_steps", "mlv_batch_profile", "mlv_batch_profile_read",
     "mlv_batch_run", "mlv_batch_job_time", "mlv_batch_tracker", "mlv_batch_autotune",
+    "mlv_batch_plan", "mlv_batch_set_plan",
     "mlv_to_spectral_basis", "mlv_to_physical_basis", "mlv_spec_op_basis",
     "mlv_fdz_work_bytes", "mlv_fdz_batch_create", "mlv_fdz_batch_destroy",
     "mlv_fdz_batch_prime", "mlv_fdz_batch_step", "mlv_fdz_batch_graph_step",
@@ -148,6 +149,8 @@ def _declare(lib):
         "mlv_spec_op_basis": ([vp, i, i, i, vp, vp, vp], i),
         "mlv_batch_run": ([vp, i, i, d, i, vp], i),
         "mlv_batch_autotune": ([vp, i, vp, ip], i),
+        "mlv_batch_plan": ([vp, ip], i),
+        "mlv_batch_set_plan": ([vp, i, i, i], i),
         "mlv_batch_job_time": ([vp, i, ctypes.POINTER(d), ctypes.POINTER(d),
                                 ctypes.POINTER(ctypes.c_longlong), ip], i),
         "mlv_batch_tracker": ([vp, i, ctypes.POINTER(d), i, ip], i),

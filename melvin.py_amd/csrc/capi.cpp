@@ -631,10 +631,13 @@ struct BatchHost {
   std::vector<const void*> keys;   // a job's identity across rebuilds: its workspace
 };
 
-// Kernel C tile shapes picked by mlv_batch_autotune, by job-set signature;
-// a batch created later for the same set of shapes starts with the choice.
+// Kernel choices picked by mlv_batch_autotune, by job-set signature; a
+// batch created later for the same set of shapes starts with them.
+struct TunedPlan {
+  int shapeA, planB, wideC;
+};
 std::mutex g_tuned_mu;
-std::map<std::string, int> g_tuned;
+std::map<std::string, TunedPlan> g_tuned;
 
 // Precision and the sorted (problem, fields, nx, nz, order) of every job:
 // what the kernels' block grids depend on.
@@ -826,7 +829,7 @@ int mlv_job_work_bytes(int problem, int nfield, int nx, int nz, int dtype, size_
 // the job set, ready for one upload (mlv_batch_create, mlv_batch_rebuild).
 struct BatchPlan {
   std::vector<char> host;  // JobDev table
-  std::vector<int> pA[2], pB[2][BV_COUNT], pC[2], pCn[2];
+  std::vector<int> pA[2][2], pB[2][BV_COUNT], pC[2], pCn[2];  // pA[shape][stage]
   BatchLaunch bl{};        // block counts and LDS sizes; pointers set by upload_batch
   size_t ldsCn = 0;
   std::vector<int> nbB;
@@ -847,7 +850,8 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
   auto& pCn = P.pCn;
   size_t& ldsCn = P.ldsCn;
   for (int s = 0; s < 2; ++s) {
-    pA[s].assign(njobs + 1, 0);
+    pA[0][s].assign(njobs + 1, 0);
+    pA[1][s].assign(njobs + 1, 0);
     pC[s].assign(njobs + 1, 0);
     pCn[s].assign(njobs + 1, 0);
     for (int v = 0; v < BV_COUNT; ++v) pB[s][v].assign(njobs + 1, 0);
@@ -855,6 +859,7 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
   BatchLaunch& bl = P.bl;
   std::memset(&bl, 0, sizeof(bl));
   bl.njobs = njobs;
+  bl.okA[0] = bl.okA[1] = 1;
   for (int j = 0; j < njobs; ++j) {
     int nstage, lognx, lognz, nm, nx;
     int nA[2], neq[2], bvar[2];
@@ -891,6 +896,10 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
     // x passes: narrow (n) and wide (w) points policies, see fused_points
     const int sla = f64 ? xinv_cols_host<double>(lognx) : xinv_cols_host<float>(lognx);
     const size_t ldsa = f64 ? xinv_lds_bytes<double>(lognx) : xinv_lds_bytes<float>(lognx);
+    // kernel A's smaller block shape, where the job's transform fits it
+    const int sla1 = f64 ? xinv_cols_host<double>(lognx, 1) : xinv_cols_host<float>(lognx, 1);
+    const size_t ldsa1 = f64 ? xinv_lds_bytes<double>(lognx, 1) : xinv_lds_bytes<float>(lognx, 1);
+    if (sla1 < 1) bl.okA[1] = 0;
     const int sln = f64 ? fused_cols_host<double>(lognx, 0) : fused_cols_host<float>(lognx, 0);
     const int slw = f64 ? fused_cols_host<double>(lognx, 1) : fused_cols_host<float>(lognx, 1);
     const size_t ldsn = f64 ? fused_lds_bytes<double>(lognx, 0) : fused_lds_bytes<float>(lognx, 0);
@@ -914,7 +923,7 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
     }
     if (nstage > P.nstage) P.nstage = nstage;
     for (int s = 0; s < 2; ++s) {
-      int a = 0, c = 0, cn = 0, bb = 0, bv = 0;
+      int a = 0, a1 = 0, c = 0, cn = 0, bb = 0, bv = 0;
       if (s < nstage) {
         const int nin_v = bvar_nin(bvar[s]), neq_v = bvar_neq(bvar[s]);
         const int slz = zslots(bvar[s]);
@@ -925,6 +934,9 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
         }
         const int rb = brows(nin_v, neq_v, slz);
         a = nA[s] * ((nm + sla - 1) / sla);
+        a1 = sla1 >= 1 ? nA[s] * ((nm + sla1 - 1) / sla1) : 0;
+        if (f64 ? bsplit_fits<double>(lognz, bvar[s]) : bsplit_fits<float>(lognz, bvar[s]))
+          bl.splitB = 1;
         c = neq[s] * ((nm + slw - 1) / slw);
         cn = neq[s] * ((nm + sln - 1) / sln);
         bb = (nx + rb - 1) / rb;
@@ -935,13 +947,15 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
         P.nbB.push_back(bb);
         P.loops0.push_back(jobs[j].loops);
       }
-      pA[s][j + 1] = pA[s][j] + a;
+      pA[0][s][j + 1] = pA[0][s][j] + a;
+      pA[1][s][j + 1] = pA[1][s][j] + a1;
       pC[s][j + 1] = pC[s][j] + c;
       pCn[s][j + 1] = pCn[s][j] + cn;
       for (int v = 0; v < BV_COUNT; ++v) pB[s][v][j + 1] = pB[s][v][j] + (v == bv ? bb : 0);
     }
     if (rc) break;
-    bl.ldsA = std::max(bl.ldsA, ldsa);
+    bl.ldsAp[0] = std::max(bl.ldsAp[0], ldsa);
+    if (sla1 >= 1) bl.ldsAp[1] = std::max(bl.ldsAp[1], ldsa1);
     // kernel M (single precision mean profiles): one LDS slot of nz
     if (!f64 && jobs[j].mean_split)
       bl.ldsM = std::max(bl.ldsM, (size_t)fft_ldsn(1 << lognz) * 8);
@@ -961,12 +975,19 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
         cus <= 0)
       cus = 256;
     bl.wideC = (dtype == MLV_F64 && pC[0][njobs] >= 4 * cus) ? 1 : 0;
-    // a shape picked by mlv_batch_autotune for the same set of jobs wins
+    // kernel A: the default block shape; kernel B: the split-pass plan
+    bl.shapeA = 0;
+    bl.planB = kPlanSplit;
+    // choices picked by mlv_batch_autotune for the same set of jobs win
     P.signature = batch_signature(dtype, jobs, njobs);
     {
       std::lock_guard<std::mutex> lk(g_tuned_mu);
       auto it = g_tuned.find(P.signature);
-      if (it != g_tuned.end()) bl.wideC = it->second;
+      if (it != g_tuned.end()) {
+        bl.shapeA = bl.okA[it->second.shapeA] ? it->second.shapeA : 0;
+        bl.planB = it->second.planB;
+        bl.wideC = it->second.wideC;
+      }
     }
     bl.ldsCp[1] = bl.ldsC;
     bl.ldsCp[0] = ldsCn;
@@ -989,7 +1010,8 @@ static int upload_batch(BatchHost* b, BatchPlan& P, const mlv_job_desc* jobs, in
   std::vector<std::pair<const std::vector<int>*, int**>> arrays;
   BatchLaunch& bl = P.bl;
   for (int s = 0; s < 2; ++s) {
-    arrays.push_back({&P.pA[s], &bl.prefA[s]});
+    arrays.push_back({&P.pA[0][s], &bl.prefAp[0][s]});
+    arrays.push_back({&P.pA[1][s], &bl.prefAp[1][s]});
     arrays.push_back({&P.pC[s], &bl.prefCp[1][s]});
     arrays.push_back({&P.pCn[s], &bl.prefCp[0][s]});
     for (int v = 0; v < BV_COUNT; ++v) arrays.push_back({&P.pB[s][v], &bl.prefB[s][v]});
@@ -1011,7 +1033,8 @@ static int upload_batch(BatchHost* b, BatchPlan& P, const mlv_job_desc* jobs, in
   for (size_t k = 0; k < arrays.size(); ++k)
     *arrays[k].second = reinterpret_cast<int*>(base + aoff[k]);
   for (int s = 0; s < 2; ++s) {
-    bl.nA[s] = P.pA[s][njobs];
+    bl.nAp[0][s] = P.pA[0][s][njobs];
+    bl.nAp[1][s] = P.pA[1][s][njobs];
     bl.nCp[1][s] = P.pC[s][njobs];
     bl.nCp[0][s] = P.pCn[s][njobs];
     for (int v = 0; v < BV_COUNT; ++v) bl.nB[s][v] = P.pB[s][v][njobs];
@@ -1073,6 +1096,7 @@ static int upload_batch(BatchHost* b, BatchPlan& P, const mlv_job_desc* jobs, in
   b->host = P.host;
   b->steps = 0;
   b->bl = bl;
+  select_a_shape(b->bl, b->bl.shapeA);
   select_c_policy(b->bl, b->bl.wideC);
   hipError_t e = b->dtype == MLV_F64 ? fused_prepare<double>(b->bl) : fused_prepare<float>(b->bl);
   if (e != hipSuccess) return hip_fail(e, "kernel attributes");
@@ -1257,54 +1281,115 @@ int mlv_batch_graph_step(mlv_batch* bp, int nsteps, void* stream) {
   return graph_run(reinterpret_cast<BatchHost*>(bp), nsteps, 0, 0.0, stream);
 }
 
+// Drop a captured graph: it holds the launches of the choices it was
+// captured with.
+static void drop_graph(BatchHost* b) {
+  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->graph_exec = nullptr;
+  b->graph = nullptr;
+}
+
+// Apply kernel choices (validated by the caller).
+static void apply_plan(BatchHost* b, int shapeA, int planB, int wideC) {
+  select_a_shape(b->bl, shapeA);
+  b->bl.planB = planB;
+  select_c_policy(b->bl, wideC);
+}
+
 int mlv_batch_autotune(mlv_batch* bp, int nsteps, void* stream, int* chosen) {
   if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
   if (nsteps < 1) return fail(MLV_ERR_ARG, "nsteps must be positive");
   BatchHost* b = reinterpret_cast<BatchHost*>(bp);
   hipStream_t s = (hipStream_t)stream;
-  hipEvent_t ev[2];
-  HIP_TRY(hipEventCreate(&ev[0]), "hipEventCreate");
-  HIP_TRY(hipEventCreate(&ev[1]), "hipEventCreate");
-  hipError_t e = hipSuccess;
-  // Segments in the order narrow, wide, wide, narrow (half of nsteps each;
-  // one segment per shape when nsteps is 1), each timed on its own; a shape's
-  // figure is its faster segment per step.  The mirrored order and the
-  // minimum keep a clock ramp after an idle period, and the one-time cost of
-  // a kernel's first launch, from deciding the choice.
+  // Options per kernel: A's block shapes (the smaller where every job fits
+  // it), B's plans (two where some job takes the split-pass plan), C's tile
+  // shapes.  Segments run every kernel at option 0, then at option 1 (a
+  // kernel with one option stays at 0), then 1 and 0 again -- half of
+  // nsteps each; one segment per option when nsteps is 1 -- with events
+  // around each kernel of every step, so each kernel's options are timed in
+  // the same steps.  A kernel's option figure is its faster segment per
+  // step: the mirrored order and the minimum keep a clock ramp after an
+  // idle period, and a kernel's first launch, from deciding the choice.
+  // The kernels read and write whole buffers in HBM between launches, so
+  // one kernel's choice does not change another's time.
+  const int nopt[3] = {b->bl.okA[1] ? 2 : 1, b->bl.splitB ? 2 : 1, 2};
   const int h1 = nsteps >= 2 ? nsteps / 2 : nsteps, h2 = nsteps - h1;
-  const int seg_shape[4] = {0, 1, 1, 0};
+  const int seg_opt[4] = {0, 1, 1, 0};
   const int seg_steps[4] = {h1, h1, h2, h2};
-  float best_ms[2] = {INFINITY, INFINITY};
+  float best_ms[3][2] = {{INFINITY, INFINITY}, {INFINITY, INFINITY}, {INFINITY, INFINITY}};
+  const int per_step = 4 * b->nstage;
+  std::vector<hipEvent_t> ev((size_t)std::max(h1, h2) * per_step, nullptr);
+  hipError_t e = hipSuccess;
+  for (auto& x : ev)
+    if (e == hipSuccess) e = hipEventCreate(&x);
   for (int k = 0; k < 4 && e == hipSuccess; ++k) {
     const int n = seg_steps[k];
     if (n == 0) continue;
-    select_c_policy(b->bl, seg_shape[k]);
-    float t = 0.f;
-    e = hipEventRecord(ev[0], s);
-    if (e == hipSuccess) e = enqueue_steps(b, n, 0, b->steps, s, false);
-    if (e == hipSuccess) e = hipEventRecord(ev[1], s);
-    if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
-    if (e == hipSuccess) e = hipEventElapsedTime(&t, ev[0], ev[1]);
-    if (e == hipSuccess) {
-      b->steps += n;
-      best_ms[seg_shape[k]] = std::min(best_ms[seg_shape[k]], t / n);
+    int opt[3];
+    for (int q = 0; q < 3; ++q) opt[q] = seg_opt[k] < nopt[q] ? seg_opt[k] : 0;
+    apply_plan(b, opt[0], opt[1], opt[2]);
+    for (int t = 0; t < n && e == hipSuccess; ++t) {
+      const int step = (int)((b->steps + t) & 3);
+      for (int st = 0; st < b->nstage && e == hipSuccess; ++st) {
+        hipEvent_t* evs = &ev[(size_t)t * per_step + 4 * st];
+        e = b->dtype == MLV_F64
+                ? fused_stage<double>((const JobDev<double>*)b->jobs_dev, b->bl, st, 0, step,
+                                      (const cx<double>*)b->tw, s, evs)
+                : fused_stage<float>((const JobDev<float>*)b->jobs_dev, b->bl, st, 0, step,
+                                     (const cx<float>*)b->tw, s, evs);
+      }
     }
+    if (e == hipSuccess) e = hipEventSynchronize(ev[(size_t)n * per_step - 1]);
+    if (e != hipSuccess) break;
+    b->steps += n;
+    float sum[3] = {0.f, 0.f, 0.f};
+    for (int t = 0; t < n * b->nstage && e == hipSuccess; ++t)
+      for (int q = 0; q < 3 && e == hipSuccess; ++q) {
+        float ms = 0.f;
+        e = hipEventElapsedTime(&ms, ev[4 * t + q], ev[4 * t + q + 1]);
+        sum[q] += ms;
+      }
+    for (int q = 0; q < 3; ++q)
+      if (seg_opt[k] < nopt[q]) best_ms[q][seg_opt[k]] = std::min(best_ms[q][seg_opt[k]], sum[q] / n);
   }
-  (void)hipEventDestroy(ev[0]);
-  (void)hipEventDestroy(ev[1]);
+  for (auto& x : ev)
+    if (x) (void)hipEventDestroy(x);
   if (e != hipSuccess) return hip_fail(e, "autotune");
-  const int best = best_ms[1] < best_ms[0] ? 1 : 0;
-  select_c_policy(b->bl, best);
-  // a captured graph holds the launches of the shape it was captured with
-  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->graph_exec = nullptr;
-  b->graph = nullptr;
+  int best[3];
+  for (int q = 0; q < 3; ++q) best[q] = (nopt[q] > 1 && best_ms[q][1] < best_ms[q][0]) ? 1 : 0;
+  apply_plan(b, best[0], best[1], best[2]);
+  drop_graph(b);
   {
     std::lock_guard<std::mutex> lk(g_tuned_mu);
-    g_tuned[b->signature] = best;
+    g_tuned[b->signature] = TunedPlan{best[0], best[1], best[2]};
   }
-  if (chosen) *chosen = best;
+  if (chosen)
+    for (int q = 0; q < 3; ++q) chosen[q] = best[q];
+  return MLV_OK;
+}
+
+int mlv_batch_plan(const mlv_batch* bp, int* plan) {
+  if (!bp || !plan) return fail(MLV_ERR_ARG, "batch or plan is NULL");
+  const BatchHost* b = reinterpret_cast<const BatchHost*>(bp);
+  plan[0] = b->bl.shapeA;
+  plan[1] = b->bl.planB;
+  plan[2] = b->bl.wideC;
+  plan[3] = b->bl.okA[1];
+  plan[4] = b->bl.splitB;
+  return MLV_OK;
+}
+
+int mlv_batch_set_plan(mlv_batch* bp, int shape_xinv, int plan_zpass, int wide_update) {
+  if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
+  BatchHost* b = reinterpret_cast<BatchHost*>(bp);
+  if (shape_xinv < 0 || shape_xinv > 1 || plan_zpass < 0 || plan_zpass > 1 || wide_update < 0 ||
+      wide_update > 1)
+    return fail(MLV_ERR_ARG, "kernel choices are 0 or 1");
+  if (shape_xinv == 1 && !b->bl.okA[1])
+    return fail(MLV_ERR_UNSUPPORTED, "some job's x transform does not fit the smaller x-inverse block");
+  apply_plan(b, shape_xinv, plan_zpass, wide_update);
+  drop_graph(b);
   return MLV_OK;
 }
 

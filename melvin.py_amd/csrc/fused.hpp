@@ -209,31 +209,34 @@ inline size_t fused_lds_bytes(int logn, int wide) {
   return (size_t)s * fft_ldsn(1 << logn) * sizeof(cx<T>);
 }
 
-// Threads per block of the x-inverse kernel A.  Single precision takes
-// 512, so that a block holds 8 columns at 1024 points and each interleaved
-// row pair's store is a whole 128-byte line (two such blocks per CU give as
-// many wavefronts as four blocks of 256); with 64-byte runs the stores of
-// neighbouring tiles only merged in L2 when their blocks ran side by side.
-// Double precision keeps 256 (4 columns of 16-byte values already fill a
-// line, and 512 would not fit the LDS at 4096 points).
-template <typename T>
+// Threads per block of the x-inverse kernel A, by shape (BatchLaunch::
+// shapeA).  Shape 0, the default: single precision takes 512, so that a
+// block holds 8 columns at 1024 points and each interleaved row pair's store
+// is a whole 128-byte line (two such blocks per CU give as many wavefronts
+// as four blocks of 256); with 64-byte runs the stores of neighbouring tiles
+// only merged in L2 when their blocks ran side by side.  Double precision
+// takes 256 (4 columns of 16-byte values already fill a line, and 512 would
+// not fit the LDS at 4096 points).  Shape 1 halves the block (half the
+// columns per block, twice the blocks); the autotuner times both.
+template <typename T, int SHAPE = 0>
 __host__ __device__ constexpr int xinv_threads() {
-  return sizeof(T) == 4 ? 512 : 256;
+  return (sizeof(T) == 4 ? 512 : 256) >> SHAPE;
 }
-template <typename T, int LOGN, int P>
+template <typename T, int LOGN, int P, int SHAPE = 0>
 __host__ __device__ constexpr int xinv_cols() {
-  return ((xinv_threads<T>() * P) >> LOGN) < xinv_threads<T>() ? ((xinv_threads<T>() * P) >> LOGN)
-                                                                : xinv_threads<T>();
+  return ((xinv_threads<T, SHAPE>() * P) >> LOGN) < xinv_threads<T, SHAPE>()
+             ? ((xinv_threads<T, SHAPE>() * P) >> LOGN)
+             : xinv_threads<T, SHAPE>();
 }
 template <typename T>
-inline int xinv_cols_host(int logn) {
-  const int s = (xinv_threads<T>() * fused_points<T>(logn, 0)) >> logn;
-  return s < xinv_threads<T>() ? s : xinv_threads<T>();
+inline int xinv_cols_host(int logn, int shape = 0) {
+  const int nt = (sizeof(T) == 4 ? 512 : 256) >> shape;
+  const int s = (nt * fused_points<T>(logn, 0)) >> logn;
+  return s < nt ? s : nt;
 }
 template <typename T>
-inline size_t xinv_lds_bytes(int logn) {
-  const int s = (xinv_threads<T>() * fused_points<T>(logn, 0)) >> logn;
-  return (size_t)s * fft_ldsn(1 << logn) * sizeof(cx<T>);
+inline size_t xinv_lds_bytes(int logn, int shape = 0) {
+  return (size_t)xinv_cols_host<T>(logn, shape) * fft_ldsn(1 << logn) * sizeof(cx<T>);
 }
 
 // Rows per z-pass block: as many as fit the LDS slots, at most 8.
@@ -263,6 +266,19 @@ __host__ __device__ constexpr int bpoints(int logn, int nin, int neq) {
 __host__ __device__ constexpr int bvar_nin(int v) { return v <= BV_VORT3 ? v + 3 : 6; }
 __host__ __device__ constexpr int bvar_neq(int v) { return v <= BV_VORT3 ? v + 1 : 1; }
 
+// The split-pass z pass (inverse 16,16,R with the products in its last
+// pass, forward R,16,16) applies at 2^9..2^11 points when the program runs
+// at 16 points per thread and fits the slots.
+template <typename T>
+__host__ __device__ constexpr bool bsplit_fits(int logn, int v) {
+  return logn >= 9 && logn <= 11 && bpoints<T>(logn, bvar_nin(v), bvar_neq(v)) == 16 &&
+         bfits(bvar_nin(v), bvar_neq(v), (kNT * 16) >> logn);
+}
+// Plans of the z pass (BatchLaunch::planB): the split-pass plan where it
+// applies, or the two-sweep plan everywhere (16,16,R both ways, the
+// products in a sweep of their own between the transforms).
+constexpr int kPlanSplit = 0, kPlanTwoSweep = 1;
+
 // workspace channel of equation e's Px (k=0) / Pz (k=1) output; the
 // z-pass inputs occupy channels [0, base)
 __host__ __device__ constexpr int pchan(int base, int e, int k) { return base + 2 * e + k; }
@@ -289,6 +305,8 @@ inline size_t fused_work_elems(int nin, int neq, int nx, int nz) {
 
 struct BatchLaunch {
   int njobs;
+  int shapeA;                          // block shape of kernel A (xinv_threads)
+  int planB;                           // plan of kernel B (kPlanSplit / kPlanTwoSweep)
   int wideC;                           // points policy of kernel C (fused_points)
   int nA[2], nB[2][BV_COUNT], nC[2];  // total blocks per launch
   size_t ldsA, ldsB[BV_COUNT], ldsC;
@@ -301,7 +319,26 @@ struct BatchLaunch {
   int nCp[2][2];
   size_t ldsCp[2];
   int* prefCp[2][2];
+  // both block shapes of kernel A (index = shape; okA[1] = 0 where some
+  // job's transform does not fit the smaller block); nA / ldsA / prefA above
+  // are the selected one
+  int okA[2];
+  int nAp[2][2];
+  size_t ldsAp[2];
+  int* prefAp[2][2];
+  // whether some job's z pass takes the split-pass plan under kPlanSplit
+  int splitB;
 };
+
+// Select kernel A's block shape.
+inline void select_a_shape(BatchLaunch& bl, int shape) {
+  bl.shapeA = shape;
+  bl.ldsA = bl.ldsAp[shape];
+  for (int s = 0; s < 2; ++s) {
+    bl.nA[s] = bl.nAp[shape][s];
+    bl.prefA[s] = bl.prefAp[shape][s];
+  }
+}
 
 // Select kernel C's tile shape (wide = 1: fused_points' wide policy).
 inline void select_c_policy(BatchLaunch& bl, int wide) {

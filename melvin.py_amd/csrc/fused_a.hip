@@ -4,15 +4,15 @@
 
 namespace mlv {
 
-template <typename T>
-__global__ void __launch_bounds__(xinv_threads<T>()) k_stepA(const JobDev<T>* __restrict__ jobs,
-                                               const int* __restrict__ pref, int njobs, int stage,
-                                               int step, const cx<T>* __restrict__ tw) {
+template <typename T, int SHAPE>
+__global__ void __launch_bounds__((xinv_threads<T, SHAPE>()))
+    k_stepA(const JobDev<T>* __restrict__ jobs, const int* __restrict__ pref, int njobs,
+            int stage, int step, const cx<T>* __restrict__ tw) {
   extern __shared__ char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int lb = xcd_logical(pref[njobs]);
   if (lb < 0) return;
-  roleA<T>(jobs, pref, njobs, lb, stage, step, tw, lds);
+  roleA<T, SHAPE>(jobs, pref, njobs, lb, stage, step, tw, lds);
 }
 
 // Grid: jobs x (kMeanCh / 2) op pairs.
@@ -31,10 +31,17 @@ template <typename T>
 void launch_xinv(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int step,
                  const cx<T>* tw, hipStream_t s) {
   if (bl.nA[stage] > 0) {
-    MLV_SET_LDS_ONCE(k_stepA<T>, bl.ldsA);
-    hipLaunchKernelGGL(k_stepA<T>, dim3(xcd_grid(bl.nA[stage])), dim3(xinv_threads<T>()), bl.ldsA,
-                       s, jobs,
-                       (const int*)bl.prefA[stage], bl.njobs, stage, step, tw);
+    const dim3 grid(xcd_grid(bl.nA[stage]));
+    const int* pref = bl.prefA[stage];
+    if (bl.shapeA == 1) {
+      MLV_SET_LDS_ONCE((k_stepA<T, 1>), bl.ldsA);
+      hipLaunchKernelGGL((k_stepA<T, 1>), grid, dim3(xinv_threads<T, 1>()), bl.ldsA, s, jobs,
+                         pref, bl.njobs, stage, step, tw);
+    } else {
+      MLV_SET_LDS_ONCE((k_stepA<T, 0>), bl.ldsA);
+      hipLaunchKernelGGL((k_stepA<T, 0>), grid, dim3(xinv_threads<T, 0>()), bl.ldsA, s, jobs,
+                         pref, bl.njobs, stage, step, tw);
+    }
   }
   if constexpr (sizeof(T) == 4) {
     if (bl.ldsM) {
@@ -47,7 +54,8 @@ void launch_xinv(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int st
 
 template <typename T>
 void prepare_xinv(const BatchLaunch& bl) {
-  if (bl.ldsA) MLV_SET_LDS_ONCE(k_stepA<T>, bl.ldsA);
+  if (bl.ldsAp[0]) MLV_SET_LDS_ONCE((k_stepA<T, 0>), bl.ldsAp[0]);
+  if (bl.okA[1] && bl.ldsAp[1]) MLV_SET_LDS_ONCE((k_stepA<T, 1>), bl.ldsAp[1]);
   if constexpr (sizeof(T) == 4) {
     if (bl.ldsM) MLV_SET_LDS_ONCE(k_stepM<T>, bl.ldsM);
   }

@@ -5,7 +5,7 @@ namespace mlv {
 
 // Single precision is held to 128 VGPRs: four wavefronts per SIMD, as many
 // as the four blocks per CU its LDS allows at 1024 points.
-template <typename T, int V, int SPLIT>
+template <typename T, int V, int SPLIT, int PLAN>
 __global__ void __launch_bounds__(kNT)
     __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
     k_stepB(const JobDev<T>* __restrict__ jobs, const int* __restrict__ pref, int njobs,
@@ -14,29 +14,38 @@ __global__ void __launch_bounds__(kNT)
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int lb = xcd_logical(pref[njobs]);
   if (lb < 0) return;
-  roleB<T, V, SPLIT>(jobs, pref, njobs, lb, stage, flags, tw, lds);
+  roleB<T, V, SPLIT, PLAN>(jobs, pref, njobs, lb, stage, flags, tw, lds);
 }
 
 // --------------------------------------------------------------------------
 // Launchers
 // --------------------------------------------------------------------------
-template <typename T, int V>
-static void launchB(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int flags,
-                    const cx<T>* tw, hipStream_t s) {
-  if (bl.nB[stage][V] == 0) return;
+template <typename T, int V, int PLAN>
+static void launchB_plan(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int flags,
+                         const cx<T>* tw, hipStream_t s) {
   const dim3 grid(xcd_grid(bl.nB[stage][V]));
   const int* pref = bl.prefB[stage][V];
   if constexpr (sizeof(T) == 4) {
     if (bl.ldsM) {  // some job carries mean profiles
-      MLV_SET_LDS_ONCE((k_stepB<T, V, 1>), bl.ldsB[V]);
-      hipLaunchKernelGGL((k_stepB<T, V, 1>), grid, dim3(kNT), bl.ldsB[V], s, jobs, pref,
+      MLV_SET_LDS_ONCE((k_stepB<T, V, 1, PLAN>), bl.ldsB[V]);
+      hipLaunchKernelGGL((k_stepB<T, V, 1, PLAN>), grid, dim3(kNT), bl.ldsB[V], s, jobs, pref,
                          bl.njobs, stage, flags, tw);
       return;
     }
   }
-  MLV_SET_LDS_ONCE((k_stepB<T, V, 0>), bl.ldsB[V]);
-  hipLaunchKernelGGL((k_stepB<T, V, 0>), grid, dim3(kNT), bl.ldsB[V], s, jobs, pref, bl.njobs,
-                     stage, flags, tw);
+  MLV_SET_LDS_ONCE((k_stepB<T, V, 0, PLAN>), bl.ldsB[V]);
+  hipLaunchKernelGGL((k_stepB<T, V, 0, PLAN>), grid, dim3(kNT), bl.ldsB[V], s, jobs, pref,
+                     bl.njobs, stage, flags, tw);
+}
+
+template <typename T, int V>
+static void launchB(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int flags,
+                    const cx<T>* tw, hipStream_t s) {
+  if (bl.nB[stage][V] == 0) return;
+  if (bl.planB == kPlanTwoSweep)
+    launchB_plan<T, V, kPlanTwoSweep>(jobs, bl, stage, flags, tw, s);
+  else
+    launchB_plan<T, V, kPlanSplit>(jobs, bl, stage, flags, tw, s);
 }
 
 template <typename T>
@@ -51,10 +60,14 @@ void launch_zpass(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int f
 
 template <typename T>
 void prepare_zpass(const BatchLaunch& bl) {
-#define MLV_B_LDS(V)                                                           \
-  if (bl.ldsB[V]) {                                                            \
-    MLV_SET_LDS_ONCE((k_stepB<T, V, 0>), bl.ldsB[V]);                          \
-    if constexpr (sizeof(T) == 4) MLV_SET_LDS_ONCE((k_stepB<T, V, 1>), bl.ldsB[V]); \
+#define MLV_B_LDS(V)                                                                     \
+  if (bl.ldsB[V]) {                                                                      \
+    MLV_SET_LDS_ONCE((k_stepB<T, V, 0, kPlanSplit>), bl.ldsB[V]);                        \
+    MLV_SET_LDS_ONCE((k_stepB<T, V, 0, kPlanTwoSweep>), bl.ldsB[V]);                     \
+    if constexpr (sizeof(T) == 4) {                                                      \
+      MLV_SET_LDS_ONCE((k_stepB<T, V, 1, kPlanSplit>), bl.ldsB[V]);                      \
+      MLV_SET_LDS_ONCE((k_stepB<T, V, 1, kPlanTwoSweep>), bl.ldsB[V]);                   \
+    }                                                                                    \
   }
   MLV_B_LDS(BV_VORT1)
   MLV_B_LDS(BV_VORT2)

@@ -1,5 +1,5 @@
 // Device bodies of the fused step's x-inverse kernels: A (spectral
-// prologue + inverse x pass, xinv_threads<T>() threads per block) and M
+// prologue + inverse x pass, xinv_threads<T, SHAPE>() threads per block) and M
 // (single-precision x-mean profiles, kNT threads per block).
 #pragma once
 #include "fused.hpp"
@@ -10,13 +10,13 @@ namespace mlv {
 // --------------------------------------------------------------------------
 // Kernel A: spectral prologue + inverse x pass
 // --------------------------------------------------------------------------
-template <typename T, int LOGN>
+template <typename T, int LOGN, int SHAPE>
 __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S, int local,
                                            int par, const cx<T>* __restrict__ tw, cx<T>* lds) {
   constexpr int N = 1 << LOGN;
   constexpr int P = fused_points<T>(LOGN, 0);
-  constexpr int NT = xinv_threads<T>();
-  constexpr int TC = xinv_cols<T, LOGN, P>();
+  constexpr int NT = xinv_threads<T, SHAPE>();
+  constexpr int TC = xinv_cols<T, LOGN, P, SHAPE>();
   constexpr int LDSN = fft_ldsn(N);
   const int nn = J.nn, nm = J.nm, rows = J.rows;
   // ops outermost: consecutive blocks write neighbouring tiles of one
@@ -131,8 +131,10 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
 }
 
 // Kernel A's work for logical block lb of a launch over the job table's
-// x-inverse blocks (prefix array pref).
-template <typename T>
+// x-inverse blocks (prefix array pref), in block shape SHAPE.  The host
+// selects a shape only when every job's transform fits its block
+// (xinv_cols_host >= 1).
+template <typename T, int SHAPE>
 __device__ __forceinline__ void roleA(const JobDev<T>* __restrict__ jobs, const int* __restrict__ pref,
                                       int njobs, int lb, int stage, int step,
                                       const cx<T>* __restrict__ tw, cx<T>* lds) {
@@ -141,9 +143,10 @@ __device__ __forceinline__ void roleA(const JobDev<T>* __restrict__ jobs, const 
   const int local = lb - pref[j];
   const int par = (J.par0 + step) & 1;
   switch (J.lognx) {
-#define X(L)                                                       \
-  case L:                                                          \
-    stepA_body<T, L>(J, J.st[stage], local, par, tw, lds);         \
+#define X(L)                                                                       \
+  case L:                                                                          \
+    if constexpr (xinv_cols<T, L, fused_points<T>(L, 0), SHAPE>() >= 1)            \
+      stepA_body<T, L, SHAPE>(J, J.st[stage], local, par, tw, lds);                \
     break;
     X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 #undef X

@@ -374,17 +374,15 @@ __device__ __forceinline__ void stepB_split(const JobDev<T>& J, const StageDev& 
   bstore<T, Sh>(J.work, chs, nm, x0, nrows * NEQ, lds);
 }
 
-// The split-pass z pass applies at 2^9..2^11 points when the program runs
-// at 16 points per thread and fits the slots.
 template <typename T, int LOGN, int V>
 __host__ __device__ constexpr bool bsplit_ok() {
-  return LOGN >= 9 && LOGN <= 11 && bpoints<T>(LOGN, bvar_nin(V), bvar_neq(V)) == 16 &&
-         bfits(bvar_nin(V), bvar_neq(V), fft_slots<LOGN, kNT, 16>());
+  static_assert(fft_slots<LOGN, kNT, 16>() == ((kNT * 16) >> LOGN), "slots at 16 points");
+  return bsplit_fits<T>(LOGN, V);
 }
 
 // Kernel B's work for logical block lb of a launch over the job table's
-// z-pass blocks of program V (prefix array pref).
-template <typename T, int V, int SPLIT>
+// z-pass blocks of program V (prefix array pref), in plan PLAN.
+template <typename T, int V, int SPLIT, int PLAN>
 __device__ __forceinline__ void roleB(const JobDev<T>* __restrict__ jobs, const int* __restrict__ pref,
                                       int njobs, int lb, int stage, int flags,
                                       const cx<T>* __restrict__ tw, cx<T>* lds) {
@@ -394,7 +392,7 @@ __device__ __forceinline__ void roleB(const JobDev<T>* __restrict__ jobs, const 
   switch (J.lognz) {
 #define X(L)                                                          \
   case L:                                                             \
-    if constexpr (bsplit_ok<T, L, V>())                               \
+    if constexpr (PLAN == kPlanSplit && bsplit_ok<T, L, V>())         \
       stepB_split<T, L, V, SPLIT>(J, J.st[stage], local, flags, tw, lds); \
     else                                                              \
       stepB_body<T, L, V, SPLIT>(J, J.st[stage], local, flags, tw, lds);  \

@@ -350,21 +350,45 @@ class Batch(DeviceRun):
         _lib.call("mlv_batch_set_dt", self.handle, int(index), float(dt),
                   _lib.stream_handle(stream))
 
-    def autotune(self, steps=4, stream=None):
-        """Pick the update kernel's tile shape by timing ``steps`` real steps
-        with each (mlv_batch_autotune); the jobs advance by 2 * steps.
-        Returns "wide" or "narrow"."""
-        chosen = ctypes.c_int()
-        before = _lib.lib().mlv_batch_steps(self.handle)
-        _lib.call("mlv_batch_autotune", self.handle, int(steps), _lib.stream_handle(stream),
-                  ctypes.byref(chosen))
-        n = _lib.lib().mlv_batch_steps(self.handle) - before
+    # names of the kernel choices (mlv_batch_autotune)
+    _SHAPE_A = ("512 threads", "256 threads")
+    _PLAN_B = ("split-pass", "two-sweep")
+    _SHAPE_C = ("narrow", "wide")
+
+    def _advance(self, n):
         for j in self.jobs:
             j.steps += n
             j.curr = (j.curr + n) % j.ab_order
             j.parity = (j.parity + n) & 1
             j.t += n * j.dt
-        return "wide" if chosen.value else "narrow"
+
+    def autotune(self, steps=4, stream=None):
+        """Pick each fused kernel's variant by timing ``steps`` real steps
+        with each option (mlv_batch_autotune); the jobs advance by 2 * steps.
+        Returns the chosen plan (see ``plan``)."""
+        chosen = (ctypes.c_int * 3)()
+        before = _lib.lib().mlv_batch_steps(self.handle)
+        _lib.call("mlv_batch_autotune", self.handle, int(steps), _lib.stream_handle(stream), chosen)
+        self._advance(_lib.lib().mlv_batch_steps(self.handle) - before)
+        return self.plan
+
+    @property
+    def plan(self):
+        """The kernel choices: x-inverse block shape, z-pass plan, update
+        tile shape, and which alternatives this job set admits."""
+        p = (ctypes.c_int * 5)()
+        _lib.call("mlv_batch_plan", self.handle, p)
+        double = self.jobs[0].dtype == "f64"
+        shape_a = ("256 threads", "128 threads") if double else self._SHAPE_A
+        return {"x_inverse": shape_a[p[0]], "z_pass": self._PLAN_B[p[1]],
+                "x_forward_update": self._SHAPE_C[p[2]],
+                "x_inverse_small_block_available": bool(p[3]),
+                "z_pass_split_available": bool(p[4])}
+
+    def set_plan(self, shape_xinv=0, plan_zpass=0, wide_update=0):
+        """Set the kernel choices (0/1 each, as in ``plan``)."""
+        _lib.call("mlv_batch_set_plan", self.handle, int(shape_xinv), int(plan_zpass),
+                  int(wide_update))
 
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:

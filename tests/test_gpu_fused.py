@@ -236,18 +236,25 @@ def test_every_program_at_2048_vs_oracle(mlv, problem, nx, nz):
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_autotune_steps_are_real_steps(mlv, dtype):
-    """mlv_batch_autotune times real steps with both tile shapes of the
-    update kernel: afterwards every job equals the same number of plain
-    steps bit for bit (the shapes compute identical values), and a batch of
-    the same job shapes created later starts with the chosen shape."""
+    """mlv_batch_autotune times real steps with both block shapes of the
+    x-inverse kernel and both tile shapes of the update kernel (the golden
+    grids are below the z pass's split-pass sizes, so its plan stays):
+    afterwards every job equals the same number of plain steps bit for bit
+    (the shapes compute identical values), and a batch of the same job
+    shapes created later starts with the chosen plan."""
     tuned = [j for j, _, _ in _golden_jobs(mlv, dtype)]
     plain = [j for j, _, _ in _golden_jobs(mlv, dtype)]
     b = mlv.fused.Batch(tuned)
     print(f"[fused] autotune {dtype}", flush=True)
     choice = b.autotune(steps=3)
-    assert choice in ("wide", "narrow")
+    assert not choice["z_pass_split_available"] and choice["z_pass"] == "split-pass"
+    assert choice["x_forward_update"] in ("wide", "narrow")
+    assert choice == b.plan
     b.step(4)
     b.close()
+    again = mlv.fused.Batch([j for j, _, _ in _golden_jobs(mlv, dtype)])
+    assert again.plan == choice
+    again.close()
     b = mlv.fused.Batch(plain)
     b.step(10)
     b.close()

@@ -87,6 +87,43 @@ def test_kh_fp32_vs_oracle(mlv, n, split):
     b.close()
 
 
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("n", [512, 1024])
+def test_kh_every_kernel_plan_vs_oracle(mlv, n, dtype):
+    """Every choice the autotuner can make (x-inverse block shape, z-pass
+    plan, update tile shape) against the oracle after 5 steps: FP64
+    <= 1e-12, FP32 <= 1e-5.  The block and tile shapes give the same bits
+    for a given z-pass plan; the two plans differ only by rounding."""
+    import itertools
+
+    p = _kh_params(n)
+    start = _kh_start(p, seed=4)
+    st = _kh_oracle(p, start, 5)
+    tol = 1e-12 if dtype == "f64" else 1e-5
+    by_plan = {}
+    for a, zp, c in itertools.product((0, 1), (0, 1), (0, 1)):
+        job = _kh_job(mlv, p, dtype, start)
+        b = mlv.fused.Batch([job])
+        avail = b.plan
+        assert avail["z_pass_split_available"]
+        if a == 1 and not avail["x_inverse_small_block_available"]:
+            b.close()
+            continue
+        print(f"[headline] KH {n}^2 {dtype} plan A{a} B{zp} C{c} x5 vs oracle", flush=True)
+        b.set_plan(a, zp, c)
+        b.step(5)
+        w, sc = job.state()
+        b.close()
+        ew, ec = rel(w, st["w"]), rel(sc, st["c"])
+        assert ew <= tol and ec <= tol, (a, zp, c, ew, ec)
+        if zp in by_plan:
+            np.testing.assert_array_equal(w, by_plan[zp][0])
+            np.testing.assert_array_equal(sc, by_plan[zp][1])
+        else:
+            by_plan[zp] = (w, sc)
+    assert set(by_plan) == {0, 1}
+
+
 def test_kh_1024_fp32_vs_fp64_1000_steps(mlv):
     """The precision target of the Kelvin-Helmholtz configuration: 1024^2,
     1000 steps, FP32 (batched, channels transformed whole as the benchmark
