@@ -5,6 +5,7 @@ moves the current slot."""
 from __future__ import annotations
 
 from . import hip
+from .Variable import regrid
 
 
 class TimeDerivative:
@@ -43,12 +44,18 @@ class TimeDerivative:
         self._data[self._curr_idx + idx] = hip.asarray(data)
 
     def load(self, data):
-        """Load a full history (order, *spectral_shape) from host data."""
-        arr = hip.asarray(data).to(self._data.dtype)
-        if tuple(arr.shape) != tuple(self._data.shape):
-            raise ValueError(f"history has shape {tuple(arr.shape)}, expected "
-                             f"{tuple(self._data.shape)}")
-        self._data.copy_(arr)
+        """Load the history, one level per integrator order
+        (TimeDerivative.py:56-62): ``data[i]`` for i < integrator_order, each
+        re-gridded onto this grid when its shape differs (Variable.regrid)."""
+        order = self._params.integrator_order
+        if len(data) < order:
+            raise ValueError(f"history has {len(data)} levels, expected {order}")
+        shape = tuple(self._data.shape[1:])
+        for i in range(order):
+            level = hip.asarray(data[i])
+            if tuple(level.shape) != shape:
+                level = regrid(self._params, level)
+            self._data[i].copy_(level.to(self._data.dtype))
 
     def get_name(self):
         return self._dump_name

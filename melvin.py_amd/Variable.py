@@ -62,8 +62,7 @@ class Variable:
             self.to_spectral()
         else:
             if tuple(data.shape) != tuple(self._sdata.shape):
-                raise ValueError(f"spectral data has shape {tuple(data.shape)}, "
-                                 f"expected {tuple(self._sdata.shape)}")
+                data = regrid(self._params, data)
             self.sets(data)
 
     def pddx(self):
@@ -115,3 +114,56 @@ class Variable:
 
     def get_name(self):
         return self._dump_name
+
+
+def scale_variable(var, outsize, xp=hip):
+    """Fully spectral coefficients ``(2 nn' + 1, nm')`` on a grid of
+    ``outsize = (nn, nm)``: shape ``(2 nn + 1, nm)``, complex, on the device
+    (Variable.py:140-151).  The coefficients are amplitudes, so each mode
+    present on both grids keeps its value -- rows ``|kx| <= min(nn', nn)``,
+    each to the row of the same kx, columns ``kz < min(nm', nm)`` -- and the
+    other modes are zero.  Unlike the reference, whose output array is real
+    (it keeps only real parts) and whose positive-row bound counts the
+    input's rows of both signs of kx (mapping negative-kx rows onto positive
+    ones when nn' < nn), the values stay complex and every kx maps to
+    itself."""
+    src = hip.asarray(var)
+    if src.ndim != 2 or src.shape[0] % 2 != 1:
+        raise ValueError(f"fully spectral data has shape (2nn+1, nm), got {tuple(src.shape)}")
+    nn, nm = int(outsize[0]), int(outsize[1])
+    dtype = src.dtype if src.is_complex() else hip.complex128
+    out = hip.zeros((2 * nn + 1, nm), dtype=dtype)
+    k = min((src.shape[0] - 1) // 2, nn)
+    m = min(src.shape[1], nm)
+    out[: k + 1, :m] = src[: k + 1, :m]
+    if k > 0:
+        out[-k:, :m] = src[-k:, :m]
+    return out
+
+
+def regrid(params, data):
+    """Spectral data of another grid size on ``params``' grid, in its
+    layout (the load-time re-grid of Variable.py:76-85 and
+    TimeDerivative.py:56-62): fully spectral by ``scale_variable``;
+    Fourier-x / FD-z ``(nn', nz)`` keeps rows ``kx < min(nn', nn)`` and
+    FD-x / Fourier-z ``(nx, nm')`` columns ``kz < min(nm', nm)``.  The
+    finite-difference axis must already match (another FD grid needs
+    interpolation, not a re-grid)."""
+    src = hip.asarray(data)
+    shape = tuple(params.spectral_shape)
+    if params.is_fully_spectral():
+        return scale_variable(src, (params.nn, params.nm)).to(hip.torch_dtype(params.complex))
+    if src.ndim != 2:
+        raise ValueError(f"spectral data must be 2-D, got {tuple(src.shape)}")
+    out = hip.zeros(shape, dtype=params.complex)
+    if params.discretisation[1] == "fdm":
+        if src.shape[1] != shape[1]:
+            raise ValueError(f"FD z grid has nz = {src.shape[1]}, expected {shape[1]}")
+        k = min(src.shape[0], shape[0])
+        out[:k] = src[:k]
+    else:
+        if src.shape[0] != shape[0]:
+            raise ValueError(f"FD x grid has nx = {src.shape[0]}, expected {shape[0]}")
+        m = min(src.shape[1], shape[1])
+        out[:, :m] = src[:, :m]
+    return out

@@ -162,6 +162,47 @@ def to_spectral(p: Params, phys: np.ndarray) -> np.ndarray:
     return out
 
 
+def regrid_spectral(p: Params, spec: np.ndarray) -> np.ndarray:
+    """Spectral coefficients of another grid size carried onto p's grid:
+    the re-grid ``Variable.load`` and ``TimeDerivative.load`` apply to
+    spectral arrays (Variable.py:76-85, ``scale_variable`` at
+    Variable.py:140-151, TimeDerivative.py:56-62).  The coefficients are
+    amplitudes (the forward transform divides by the point count), so each
+    mode present on both grids keeps its value and the others are zero:
+
+    * fully spectral ``(2 nn' + 1, nm')`` -> ``(2 nn + 1, nm)``: modes
+      ``|kx| <= min(nn', nn)``, ``kz < min(nm', nm)``, each row mapped to
+      the row of the same kx (positive rows from the top, negative from
+      the bottom);
+    * Fourier-x / FD-z ``(nn', nz)`` -> ``(nn, nz)``: rows ``kx <
+      min(nn', nn)``; the z grid must match (a different FD grid needs
+      interpolation, not a re-grid).
+
+    Two behaviours of the reference are not restated: its output array is
+    real (``xp.zeros`` defaults to float64), so the assignment keeps only
+    the real parts; and it bounds the positive rows by ``min(2 nn' + 1, nn)``
+    -- the input's row count over both signs of kx -- which, when nn' < nn,
+    copies negative-kx rows onto positive ones.  The restatement keeps the
+    complex values and maps every kx to itself.
+    """
+    spec = np.asarray(spec)
+    out = np.zeros(p.spectral_shape, dtype=np.complex128)
+    if p.fully_spectral:
+        if spec.ndim != 2 or spec.shape[0] % 2 != 1:
+            raise ValueError(f"fully spectral data has shape (2nn+1, nm), got {spec.shape}")
+        k = min((spec.shape[0] - 1) // 2, p.nn)
+        m = min(spec.shape[1], p.nm)
+        out[: k + 1, :m] = spec[: k + 1, :m]
+        if k > 0:
+            out[-k:, :m] = spec[-k:, :m]
+        return out
+    if spec.ndim != 2 or spec.shape[1] != p.nz:
+        raise ValueError(f"Fourier-x / FD-z data needs nz = {p.nz} columns, got {spec.shape}")
+    k = min(spec.shape[0], p.nn)
+    out[:k] = spec[:k]
+    return out
+
+
 # --------------------------------------------------------------------------
 # Cosine and sine bases (SpectralTransformer.py:90-199, BasisFunctions.py)
 # --------------------------------------------------------------------------

@@ -121,3 +121,45 @@ def test_init_var_with_noise_matches_reference_rule(mlv, disc):
     mlv.utility.init_var_with_noise(v, 1e-3)
     assert rel(_h(v.gets()), O.to_spectral(O.make_params(params),
                                            O.noise((64, nz), 1e-3, 0))) <= 1e-14
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+@pytest.mark.parametrize("disc", [["spectral", "spectral"], ["spectral", "fdm"]])
+def test_load_regrids_spectral_data(mlv, disc, precision):
+    """Variable.load and TimeDerivative.load carry spectral data of another
+    grid size onto the simulation's grid (reference Variable.py:76-85,
+    140-151, TimeDerivative.py:56-62): the oracle's regrid_spectral, bit for
+    bit, from coarser and from finer grids; data of the right shape loads
+    unchanged; a different FD grid is refused."""
+    from oracle import melvin_oracle as O
+
+    fdm = disc[1] == "fdm"
+    nz = 33 if fdm else 64
+    params = dict(nx=64, nz=nz, lx=2.0, lz=1.0, final_time=1.0, discretisation=disc,
+                  precision=precision, integrator_order=4)
+    p = mlv.Parameters(params)
+    sim = mlv.Simulation(p, mlv.hip)
+    bfs = [mlv.BasisFunctions.COMPLEX_EXP,
+           mlv.BasisFunctions.FDM if fdm else mlv.BasisFunctions.COMPLEX_EXP]
+    v = sim.make_variable("v", bfs)
+    d = sim.make_derivative("dv")
+    op = O.make_params(params)
+    cplx = np.complex128 if precision == "double" else np.complex64
+    rng = np.random.default_rng(7)
+    for nx_src, nz_src in ((48, 48), (96, 128), (64, 64)):
+        src_p = O.make_params(dict(nx=nx_src, nz=nz if fdm else nz_src, lx=2.0, lz=1.0,
+                                   discretisation=disc))
+        shape = src_p.spectral_shape
+        src = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        print(f"[mirror] load {disc} {precision} from {shape}", flush=True)
+        v.load(src)
+        want = O.regrid_spectral(op, src).astype(cplx)
+        np.testing.assert_array_equal(_h(v.gets()), want)
+        levels = [src * (k + 1) for k in range(4)]
+        d.load(levels)
+        for k in range(4):
+            np.testing.assert_array_equal(_h(d.get_all()[k]),
+                                          O.regrid_spectral(op, levels[k]).astype(cplx))
+    if fdm:
+        with pytest.raises(ValueError):
+            v.load(np.zeros((10, nz + 2), dtype=complex))
