@@ -106,6 +106,23 @@ int mlv_plan_info(const mlv_plan* plan, int* nn, int* nm, int* spec_rows, int* s
 int mlv_to_physical(mlv_plan* plan, const void* spec, void* phys, void* stream);
 /* SpectralTransformer.to_spectral (SpectralTransformer.py:63-88, 152-199) */
 int mlv_to_spectral(mlv_plan* plan, const void* phys, void* spec, void* stream);
+/* numpy.fft real transforms of a row-major (nx, nz) array, the xp.fft
+ * calls the reference's transformer makes (SpectralTransformer.py:58 irfft,
+ * 85 rfft, 132 irfft2, 191 rfft2).  axes: MLV_FFT_AXIS0 along x (rfft:
+ * real (nx, nz) -> complex (nx/2+1, nz)), MLV_FFT_AXIS1 along z (real
+ * (nx, nz) -> complex (nx, nz/2+1)), MLV_FFT_AXES01 both (rfft2: -> complex
+ * (nx, nz/2+1), kx in FFT order; `work` holds nx * (nz/2+1) complex
+ * values).  Forward unscaled; mlv_irfft is the inverse with numpy's 1/n
+ * scaling, nx and nz the REAL array's dimensions, and ignores the
+ * imaginary parts of the DC and Nyquist bins along the real axis as numpy
+ * does.  Transformed lengths: powers of two in [8, 4096]. */
+#define MLV_FFT_AXIS0 0
+#define MLV_FFT_AXIS1 1
+#define MLV_FFT_AXES01 2
+int mlv_rfft(int dtype, int axes, int nx, int nz, const void* in, void* out, void* work,
+             void* stream);
+int mlv_irfft(int dtype, int axes, int nx, int nz, const void* in, void* out, void* work,
+              void* stream);
 /* SpatialDifferentiator sddx/sddz/sd2dx2/sd2dz2/calc_lap, Variable.snabla2
  * and the fully spectral LaplacianSolver.solve; `in` is spectral, `out` is
  * spectral (complex) or, for MLV_SPEC_LAP, float64 with `in` ignored.  In

@@ -21,6 +21,7 @@ FULLY_SPECTRAL = 0
 FOURIER_X_FD_Z = 1
 F32 = 0
 F64 = 1
+FFT_AXIS0, FFT_AXIS1, FFT_AXES01 = 0, 1, 2
 
 SPEC_DDX, SPEC_DDZ, SPEC_D2DX2, SPEC_D2DZ2, SPEC_LAP, SPEC_SNABLA2, SPEC_POISSON = range(7)
 FD_DDX, FD_DDZ, FD_D2DZ2 = range(3)
@@ -38,7 +39,7 @@ EXPORTS = (
     "mlv_batch_destroy", "mlv_batch_step", "mlv_batch_graph_step", "mlv_batch_set_dt",
     "mlv_batch_steps", "mlv_batch_profile", "mlv_batch_profile_read",
     "mlv_batch_run", "mlv_batch_job_time", "mlv_batch_tracker", "mlv_batch_autotune",
-    "mlv_batch_plan", "mlv_batch_set_plan",
+    "mlv_batch_plan", "mlv_batch_set_plan", "mlv_rfft", "mlv_irfft",
     "mlv_to_spectral_basis", "mlv_to_physical_basis", "mlv_spec_op_basis",
     "mlv_fdz_work_bytes", "mlv_fdz_batch_create", "mlv_fdz_batch_destroy",
     "mlv_fdz_batch_prime", "mlv_fdz_batch_step", "mlv_fdz_batch_graph_step",
@@ -150,6 +151,8 @@ def _declare(lib):
         "mlv_batch_run": ([vp, i, i, d, i, vp], i),
         "mlv_batch_autotune": ([vp, i, vp, ip], i),
         "mlv_batch_plan": ([vp, ip], i),
+        "mlv_rfft": ([i, i, i, i, vp, vp, vp, vp], i),
+        "mlv_irfft": ([i, i, i, i, vp, vp, vp, vp], i),
         "mlv_batch_set_plan": ([vp, i, i, i], i),
         "mlv_batch_job_time": ([vp, i, ctypes.POINTER(d), ctypes.POINTER(d),
                                 ctypes.POINTER(ctypes.c_longlong), ip], i),

@@ -376,6 +376,103 @@ int mlv_to_spectral(mlv_plan* p, const void* phys, void* spec, void* stream) {
   return MLV_OK;
 }
 
+// numpy.fft's real transforms on a row-major (nx, nz) array (the xp.fft
+// calls of SpectralTransformer.py:58,85,132,191): the transformed axes'
+// lengths must be powers of two from 8 to 4096 (the plans' range).
+static int fft_axis_log(int n, const char* what, int* logn) {
+  int l = 0;
+  while (l < 30 && (1 << l) < n) ++l;
+  if (n < 8 || (1 << l) != n || l > 12)
+    return fail(MLV_ERR_UNSUPPORTED, std::string(what) + " must be a power of two in [8, 4096], got " +
+                                         std::to_string(n));
+  *logn = l;
+  return MLV_OK;
+}
+
+static int fft_setup(int dtype, int axes, int nx, int nz, int* lognx, int* lognz,
+                     const void** tw) {
+  if (dtype != MLV_F32 && dtype != MLV_F64) return fail(MLV_ERR_ARG, "bad dtype");
+  if (axes < MLV_FFT_AXIS0 || axes > MLV_FFT_AXES01) return fail(MLV_ERR_ARG, "bad axes");
+  if (nx < 1 || nz < 1) return fail(MLV_ERR_ARG, "empty array");
+  *lognx = *lognz = 0;
+  int rc = MLV_OK;
+  if (axes != MLV_FFT_AXIS1) rc = fft_axis_log(nx, "nx", lognx);
+  if (!rc && axes != MLV_FFT_AXIS0) rc = fft_axis_log(nz, "nz", lognz);
+  if (rc) return rc;
+  int device = 0;
+  HIP_TRY(hipGetDevice(&device), "hipGetDevice");
+  const bool f64 = dtype == MLV_F64;
+  size_t lds = 0;
+  if (*lognx) lds = f64 ? fft_lds_bytes<double>(*lognx) : fft_lds_bytes<float>(*lognx);
+  if (*lognz)
+    lds = std::max(lds, f64 ? fft_lds_bytes_row<double>(*lognz) : fft_lds_bytes_row<float>(*lognz));
+  if (lds > max_lds_per_block(device)) return fail(MLV_ERR_UNSUPPORTED, "transform too long for LDS");
+  return get_twiddles(device, dtype, tw);
+}
+
+int mlv_rfft(int dtype, int axes, int nx, int nz, const void* in, void* out, void* work,
+             void* stream) {
+  CHECK_PTR(in);
+  CHECK_PTR(out);
+  int lx, lz;
+  const void* tw = nullptr;
+  int rc = fft_setup(dtype, axes, nx, nz, &lx, &lz, &tw);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const bool f64 = dtype == MLV_F64;
+  hipError_t e = hipSuccess;
+  if (axes == MLV_FFT_AXIS0) {
+    e = f64 ? launch_xr2c_cols<double>(lx, in, out, nx / 2 + 1, nz, 1.0, tw, s)
+            : launch_xr2c_cols<float>(lx, in, out, nx / 2 + 1, nz, 1.0, tw, s);
+  } else if (axes == MLV_FFT_AXIS1) {
+    e = f64 ? launch_zr2c<double>(lz, in, out, nx, nz / 2 + 1, tw, s)
+            : launch_zr2c<float>(lz, in, out, nx, nz / 2 + 1, tw, s);
+  } else {
+    CHECK_PTR(work);
+    const int nm = nz / 2 + 1;
+    e = f64 ? launch_zr2c<double>(lz, in, work, nx, nm, tw, s)
+            : launch_zr2c<float>(lz, in, work, nx, nm, tw, s);
+    if (e == hipSuccess)
+      e = f64 ? launch_xfwd<double>(lx, work, out, nx / 2, nm, 1.0, tw, s, nx)
+              : launch_xfwd<float>(lx, work, out, nx / 2, nm, 1.0, tw, s, nx);
+  }
+  if (e != hipSuccess) return hip_fail(e, "rfft");
+  return MLV_OK;
+}
+
+int mlv_irfft(int dtype, int axes, int nx, int nz, const void* in, void* out, void* work,
+              void* stream) {
+  CHECK_PTR(in);
+  CHECK_PTR(out);
+  int lx, lz;
+  const void* tw = nullptr;
+  int rc = fft_setup(dtype, axes, nx, nz, &lx, &lz, &tw);
+  if (rc) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const bool f64 = dtype == MLV_F64;
+  hipError_t e = hipSuccess;
+  if (axes == MLV_FFT_AXIS0) {
+    const double sc = 1.0 / nx;
+    e = f64 ? launch_xc2r_cols<double>(lx, in, out, nx / 2 + 1, nz, tw, s, sc)
+            : launch_xc2r_cols<float>(lx, in, out, nx / 2 + 1, nz, tw, s, sc);
+  } else if (axes == MLV_FFT_AXIS1) {
+    const double sc = 1.0 / nz;
+    e = f64 ? launch_zc2r<double>(lz, in, out, nx, nz / 2 + 1, tw, s, sc)
+            : launch_zc2r<float>(lz, in, out, nx, nz / 2 + 1, tw, s, sc);
+  } else {
+    CHECK_PTR(work);
+    const int nm = nz / 2 + 1;
+    const double sc = 1.0 / ((double)nx * (double)nz);
+    e = f64 ? launch_xinv<double>(lx, in, work, nx / 2, nm, tw, s, nx, sc)
+            : launch_xinv<float>(lx, in, work, nx / 2, nm, tw, s, nx, sc);
+    if (e == hipSuccess)
+      e = f64 ? launch_zc2r<double>(lz, work, out, nx, nm, tw, s)
+              : launch_zc2r<float>(lz, work, out, nx, nm, tw, s);
+  }
+  if (e != hipSuccess) return hip_fail(e, "irfft");
+  return MLV_OK;
+}
+
 int mlv_spec_op(mlv_plan* p, int kind, const void* in, void* out, void* stream) {
   CHECK_PLAN(p);
   CHECK_PTR(out);

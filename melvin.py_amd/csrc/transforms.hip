@@ -13,7 +13,12 @@
 //
 // Real transforms pack two real sequences into one complex transform
 // (rows x, x+1 for the z pass; columns z, z+1 for the FD-z x pass).  As in
-// numpy's c2r, the imaginary part of each sequence's DC bin is ignored.
+// numpy's c2r, the imaginary parts of each sequence's DC bin and, when the
+// input holds it, its Nyquist bin are ignored.
+//
+// The same kernels give numpy.fft's rfft / irfft / rfft2 / irfft2 (full
+// half-spectra, unnormalised forward, 1/n inverse): nm = nz/2 + 1 or
+// nn = nx/2 + 1 bins, and the x passes over all nx rows (nrow = nx).
 #include "transforms.hpp"
 
 namespace mlv {
@@ -23,8 +28,8 @@ namespace mlv {
 // --------------------------------------------------------------------------
 template <typename T, int LOGN>
 __global__ void __launch_bounds__(kNT) k_xinv(const cx<T>* __restrict__ in,
-                                              cx<T>* __restrict__ out, int nn, int nm,
-                                              const cx<T>* __restrict__ tw) {
+                                              cx<T>* __restrict__ out, int nn, int nrow, int nm,
+                                              T scale, const cx<T>* __restrict__ tw) {
   constexpr int N = 1 << LOGN;
   constexpr int P = fft_points_col<T>(LOGN);
   constexpr int TC = fft_slots<LOGN, kNT, P>();
@@ -33,13 +38,12 @@ __global__ void __launch_bounds__(kNT) k_xinv(const cx<T>* __restrict__ in,
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int c0 = blockIdx.x * TC;
   const int ncol = min(TC, nm - c0);
-  const int nrow = 2 * nn + 1;
   for (int e = threadIdx.x; e < N * TC; e += kNT) {
     const int c = e % TC, x = e / TC;
     cx<T> v = czero<T>();
     if (c < ncol) {
       const int r = x <= nn ? x : (x >= N - nn ? x - N + nrow : -1);
-      if (r >= 0) v = in[(size_t)r * nm + c0 + c];
+      if (r >= 0) v = cscale<T>(in[(size_t)r * nm + c0 + c], scale);
     }
     lds[c * LDSN + fft_pad(x)] = v;
   }
@@ -56,7 +60,7 @@ __global__ void __launch_bounds__(kNT) k_xinv(const cx<T>* __restrict__ in,
 // --------------------------------------------------------------------------
 template <typename T, int LOGN>
 __global__ void __launch_bounds__(kNT) k_xfwd(const cx<T>* __restrict__ in,
-                                              cx<T>* __restrict__ out, int nn, int nm,
+                                              cx<T>* __restrict__ out, int nn, int nrow, int nm,
                                               T scale, const cx<T>* __restrict__ tw) {
   constexpr int N = 1 << LOGN;
   constexpr int P = fft_points_col<T>(LOGN);
@@ -66,7 +70,6 @@ __global__ void __launch_bounds__(kNT) k_xfwd(const cx<T>* __restrict__ in,
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int c0 = blockIdx.x * TC;
   const int ncol = min(TC, nm - c0);
-  const int nrow = 2 * nn + 1;
   for (int e = threadIdx.x; e < N * TC; e += kNT) {
     const int c = e % TC, x = e / TC;
     cx<T> v = czero<T>();
@@ -89,7 +92,7 @@ __global__ void __launch_bounds__(kNT) k_xfwd(const cx<T>* __restrict__ in,
 // --------------------------------------------------------------------------
 template <typename T, int LOGN>
 __global__ void __launch_bounds__(kNT) k_zc2r(const cx<T>* __restrict__ in,
-                                              T* __restrict__ out, int nx, int nm,
+                                              T* __restrict__ out, int nx, int nm, T scale,
                                               const cx<T>* __restrict__ tw) {
   constexpr int N = 1 << LOGN;  // nz
   constexpr int P = fft_points_row<T>(LOGN);
@@ -111,8 +114,8 @@ __global__ void __launch_bounds__(kNT) k_zc2r(const cx<T>* __restrict__ in,
     const cx<T> a = in[(size_t)xa * nm + k];
     const cx<T> b = xb < nx ? in[(size_t)xb * nm + k] : czero<T>();
     cx<T>* z = lds + s * LDSN;
-    if (k == 0) {
-      z[0] = mk<T>(a.x, b.x);
+    if (k == 0 || 2 * k == N) {
+      z[fft_pad(k)] = mk<T>(a.x, b.x);
     } else {
       z[fft_pad(k)] = mk<T>(a.x - b.y, a.y + b.x);
       z[fft_pad(N - k)] = mk<T>(a.x + b.y, b.x - a.y);
@@ -124,8 +127,8 @@ __global__ void __launch_bounds__(kNT) k_zc2r(const cx<T>* __restrict__ in,
     const int s = e / N, n = e % N;
     const int xa = x0 + 2 * s, xb = xa + 1;
     const cx<T> v = lds[s * LDSN + fft_pad(n)];
-    out[(size_t)xa * N + n] = v.x;
-    if (xb < nx) out[(size_t)xb * N + n] = v.y;
+    out[(size_t)xa * N + n] = v.x * scale;
+    if (xb < nx) out[(size_t)xb * N + n] = v.y * scale;
   }
 }
 
@@ -169,7 +172,7 @@ __global__ void __launch_bounds__(kNT) k_zr2c(const T* __restrict__ in,
 // --------------------------------------------------------------------------
 template <typename T, int LOGN>
 __global__ void __launch_bounds__(kNT) k_xc2r_cols(const cx<T>* __restrict__ in,
-                                                   T* __restrict__ out, int nn, int nz,
+                                                   T* __restrict__ out, int nn, int nz, T scale,
                                                    const cx<T>* __restrict__ tw) {
   constexpr int N = 1 << LOGN;  // nx
   constexpr int P = fft_points_col<T>(LOGN);
@@ -191,8 +194,8 @@ __global__ void __launch_bounds__(kNT) k_xc2r_cols(const cx<T>* __restrict__ in,
     const cx<T> a = in[(size_t)k * nz + za];
     const cx<T> b = zb < nz ? in[(size_t)k * nz + zb] : czero<T>();
     cx<T>* q = lds + s * LDSN;
-    if (k == 0) {
-      q[0] = mk<T>(a.x, b.x);
+    if (k == 0 || 2 * k == N) {
+      q[fft_pad(k)] = mk<T>(a.x, b.x);
     } else {
       q[fft_pad(k)] = mk<T>(a.x - b.y, a.y + b.x);
       q[fft_pad(N - k)] = mk<T>(a.x + b.y, b.x - a.y);
@@ -205,7 +208,7 @@ __global__ void __launch_bounds__(kNT) k_xc2r_cols(const cx<T>* __restrict__ in,
     const int z = z0 + zc;
     if (z < nz) {
       const cx<T> v = lds[(zc >> 1) * LDSN + fft_pad(x)];
-      out[(size_t)x * nz + z] = (zc & 1) ? v.y : v.x;
+      out[(size_t)x * nz + z] = ((zc & 1) ? v.y : v.x) * scale;
     }
   }
 }
@@ -264,7 +267,8 @@ __global__ void __launch_bounds__(kNT) k_xr2c_cols(const T* __restrict__ in,
 
 template <typename T>
 hipError_t launch_xinv(int logn, const void* in, void* out, int nn, int nm, const void* tw,
-                       hipStream_t s) {
+                       hipStream_t s, int nrow, double scale) {
+  if (nrow < 0) nrow = 2 * nn + 1;
   const int tc = slots_for<T>(logn);
   const dim3 grid((nm + tc - 1) / tc);
   const size_t lds = fft_lds_bytes<T>(logn);
@@ -273,7 +277,7 @@ hipError_t launch_xinv(int logn, const void* in, void* out, int nn, int nm, cons
   case L:                                                                                  \
     MLV_SET_LDS((k_xinv<T, L>), lds);                                                      \
     hipLaunchKernelGGL((k_xinv<T, L>), grid, dim3(kNT), lds, s, (const cx<T>*)in,          \
-                       (cx<T>*)out, nn, nm, (const cx<T>*)tw);                             \
+                       (cx<T>*)out, nn, nrow, nm, (T)scale, (const cx<T>*)tw);             \
     break;
     MLV_LOGN_CASES(X)
 #undef X
@@ -285,7 +289,8 @@ hipError_t launch_xinv(int logn, const void* in, void* out, int nn, int nm, cons
 
 template <typename T>
 hipError_t launch_xfwd(int logn, const void* in, void* out, int nn, int nm, double scale,
-                       const void* tw, hipStream_t s) {
+                       const void* tw, hipStream_t s, int nrow) {
+  if (nrow < 0) nrow = 2 * nn + 1;
   const int tc = slots_for<T>(logn);
   const dim3 grid((nm + tc - 1) / tc);
   const size_t lds = fft_lds_bytes<T>(logn);
@@ -294,7 +299,7 @@ hipError_t launch_xfwd(int logn, const void* in, void* out, int nn, int nm, doub
   case L:                                                                                  \
     MLV_SET_LDS((k_xfwd<T, L>), lds);                                                      \
     hipLaunchKernelGGL((k_xfwd<T, L>), grid, dim3(kNT), lds, s, (const cx<T>*)in,          \
-                       (cx<T>*)out, nn, nm, (T)scale, (const cx<T>*)tw);                   \
+                       (cx<T>*)out, nn, nrow, nm, (T)scale, (const cx<T>*)tw);             \
     break;
     MLV_LOGN_CASES(X)
 #undef X
@@ -306,7 +311,7 @@ hipError_t launch_xfwd(int logn, const void* in, void* out, int nn, int nm, doub
 
 template <typename T>
 hipError_t launch_zc2r(int logn, const void* in, void* out, int nx, int nm, const void* tw,
-                       hipStream_t s) {
+                       hipStream_t s, double scale) {
   const int rows = 2 * slots_row<T>(logn);
   const dim3 grid((nx + rows - 1) / rows);
   const size_t lds = fft_lds_bytes_row<T>(logn);
@@ -315,7 +320,7 @@ hipError_t launch_zc2r(int logn, const void* in, void* out, int nx, int nm, cons
   case L:                                                                                  \
     MLV_SET_LDS((k_zc2r<T, L>), lds);                                                      \
     hipLaunchKernelGGL((k_zc2r<T, L>), grid, dim3(kNT), lds, s, (const cx<T>*)in, (T*)out, \
-                       nx, nm, (const cx<T>*)tw);                                          \
+                       nx, nm, (T)scale, (const cx<T>*)tw);                                \
     break;
     MLV_LOGN_CASES(X)
 #undef X
@@ -348,7 +353,7 @@ hipError_t launch_zr2c(int logn, const void* in, void* out, int nx, int nm, cons
 
 template <typename T>
 hipError_t launch_xc2r_cols(int logn, const void* in, void* out, int nn, int nz,
-                            const void* tw, hipStream_t s) {
+                            const void* tw, hipStream_t s, double scale) {
   const int cols = 2 * slots_for<T>(logn);
   const dim3 grid((nz + cols - 1) / cols);
   const size_t lds = fft_lds_bytes<T>(logn);
@@ -357,7 +362,7 @@ hipError_t launch_xc2r_cols(int logn, const void* in, void* out, int nn, int nz,
   case L:                                                                                  \
     MLV_SET_LDS((k_xc2r_cols<T, L>), lds);                                                 \
     hipLaunchKernelGGL((k_xc2r_cols<T, L>), grid, dim3(kNT), lds, s, (const cx<T>*)in,     \
-                       (T*)out, nn, nz, (const cx<T>*)tw);                                 \
+                       (T*)out, nn, nz, (T)scale, (const cx<T>*)tw);                       \
     break;
     MLV_LOGN_CASES(X)
 #undef X
@@ -390,15 +395,15 @@ hipError_t launch_xr2c_cols(int logn, const void* in, void* out, int nn, int nz,
 
 #define MLV_INST(T)                                                                         \
   template hipError_t launch_xinv<T>(int, const void*, void*, int, int, const void*,        \
-                                     hipStream_t);                                          \
+                                     hipStream_t, int, double);                             \
   template hipError_t launch_xfwd<T>(int, const void*, void*, int, int, double,             \
-                                     const void*, hipStream_t);                             \
+                                     const void*, hipStream_t, int);                        \
   template hipError_t launch_zc2r<T>(int, const void*, void*, int, int, const void*,        \
-                                     hipStream_t);                                          \
+                                     hipStream_t, double);                                  \
   template hipError_t launch_zr2c<T>(int, const void*, void*, int, int, const void*,        \
                                      hipStream_t);                                          \
   template hipError_t launch_xc2r_cols<T>(int, const void*, void*, int, int, const void*,   \
-                                          hipStream_t);                                     \
+                                          hipStream_t, double);                             \
   template hipError_t launch_xr2c_cols<T>(int, const void*, void*, int, int, double,        \
                                           const void*, hipStream_t);
 MLV_INST(float)

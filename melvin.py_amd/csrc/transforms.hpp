@@ -27,21 +27,24 @@ inline size_t fft_lds_bytes_row(int logn) {
   return (size_t)slots_row<T>(logn) * fft_ldsn(1 << logn) * sizeof(cx<T>);
 }
 
+// nrow: rows of the x-pass spectrum (default 2 nn + 1, the compact layout;
+// nx for a full spectrum, rows in FFT order); scale multiplies the input
+// of launch_xinv and the outputs of the c2r passes.
 template <typename T>
 hipError_t launch_xinv(int logn, const void* in, void* out, int nn, int nm, const void* tw,
-                       hipStream_t s);
+                       hipStream_t s, int nrow = -1, double scale = 1.0);
 template <typename T>
 hipError_t launch_xfwd(int logn, const void* in, void* out, int nn, int nm, double scale,
-                       const void* tw, hipStream_t s);
+                       const void* tw, hipStream_t s, int nrow = -1);
 template <typename T>
 hipError_t launch_zc2r(int logn, const void* in, void* out, int nx, int nm, const void* tw,
-                       hipStream_t s);
+                       hipStream_t s, double scale = 1.0);
 template <typename T>
 hipError_t launch_zr2c(int logn, const void* in, void* out, int nx, int nm, const void* tw,
                        hipStream_t s);
 template <typename T>
 hipError_t launch_xc2r_cols(int logn, const void* in, void* out, int nn, int nz,
-                            const void* tw, hipStream_t s);
+                            const void* tw, hipStream_t s, double scale = 1.0);
 template <typename T>
 hipError_t launch_xr2c_cols(int logn, const void* in, void* out, int nn, int nz,
                             double scale, const void* tw, hipStream_t s);

@@ -72,6 +72,17 @@ def test_argument_errors_are_reported():
     assert lib.mlv_spec_op_basis(None, 0, 1, 1, None, None, None) == L.MLV_ERR_ARG
     assert lib.mlv_batch_step(None, 1, 0, None) == L.MLV_ERR_ARG
     assert lib.mlv_batch_steps(None) == -1
+    assert lib.mlv_batch_plan(None, None) == L.MLV_ERR_ARG
+    assert lib.mlv_batch_set_plan(None, 0, 0, 0) == L.MLV_ERR_ARG
+    # xp.fft entry points: argument checks come before any device call
+    buf = ctypes.c_void_p(1)
+    assert lib.mlv_rfft(0, 2, 32, 32, None, buf, buf, None) == L.MLV_ERR_ARG
+    assert lib.mlv_rfft(0, 3, 32, 32, buf, buf, buf, None) == L.MLV_ERR_ARG
+    assert lib.mlv_rfft(2, 2, 32, 32, buf, buf, buf, None) == L.MLV_ERR_ARG
+    assert lib.mlv_rfft(0, 0, 48, 33, buf, buf, None, None) == L.MLV_ERR_UNSUPPORTED
+    assert b"nx must be a power of two" in lib.mlv_last_error()
+    assert lib.mlv_irfft(1, 1, 33, 2, buf, buf, None, None) == L.MLV_ERR_UNSUPPORTED
+    assert lib.mlv_irfft(1, 2, 16, 8192, buf, buf, buf, None) == L.MLV_ERR_UNSUPPORTED
     n = ctypes.c_size_t()
     assert lib.mlv_job_work_bytes(9, 1, 32, 32, 0, ctypes.byref(n)) == L.MLV_ERR_ARG
     assert lib.mlv_job_work_bytes(L.PROB_VORTICITY, 1, 32, 32, 0, ctypes.byref(n)) == 0
