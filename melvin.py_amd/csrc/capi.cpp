@@ -957,6 +957,7 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
   std::memset(&bl, 0, sizeof(bl));
   bl.njobs = njobs;
   bl.okA[0] = bl.okA[1] = 1;
+  bl.lognx_all = bl.lognz_all = 0;
   for (int j = 0; j < njobs; ++j) {
     int nstage, lognx, lognz, nm, nx;
     int nA[2], neq[2], bvar[2];
@@ -990,6 +991,8 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
       }
     }
     const bool f64 = dtype == MLV_F64;
+    bl.lognx_all = (j == 0 || bl.lognx_all == lognx) ? lognx : -1;
+    bl.lognz_all = (j == 0 || bl.lognz_all == lognz) ? lognz : -1;
     // x passes: narrow (n) and wide (w) points policies, see fused_points
     const int sla = f64 ? xinv_cols_host<double>(lognx) : xinv_cols_host<float>(lognx);
     const size_t ldsa = f64 ? xinv_lds_bytes<double>(lognx) : xinv_lds_bytes<float>(lognx);

@@ -347,8 +347,8 @@ __device__ void fdz_row(const FdzJob<T>& J, int stage, int n, int f, int step,
 
 template <typename T, int KT>
 __global__ void __launch_bounds__(64 * kFdzWaves)
-    k_fdz_rows(const FdzJob<T>* __restrict__ jobs, const int* __restrict__ pref, int njobs,
-               int phase, int step, int zstride) {
+    k_fdz_rows(const FdzJob<T>* __restrict__ jobs, FdzJob<T>* clock, const int* __restrict__ pref,
+               int njobs, int phase, int step, int zstride) {
   extern __shared__ char smem_raw[];
   const int lb = xcd_logical(pref[njobs]);
   if (lb < 0) return;
@@ -365,11 +365,11 @@ __global__ void __launch_bounds__(64 * kFdzWaves)
     nwf = 2;
   }
   if (phase == (J.scheme == MLV_FDZ_PC ? 1 : 0) && lb == pref[jb] && threadIdx.x == 0) {
-    // t += dt after each loop (Simulation.py:199-209); nothing else in this
-    // launch reads these two fields
-    FdzJob<T>* Jw = const_cast<FdzJob<T>*>(jobs) + jb;
-    Jw->t += J.dt;
-    Jw->loops += 1;
+    // t += dt after each loop (Simulation.py:199-209), through `clock` (the
+    // same table as a plain pointer); this launch reads these two fields
+    // through neither pointer
+    clock[jb].t = clock[jb].t + J.dt;
+    clock[jb].loops = clock[jb].loops + 1;
   }
   if (lw >= nwf * J.nn || J.kt != KT) return;
   const int n = lw / nwf, f = lw % nwf;
@@ -692,7 +692,8 @@ static void launch_rows(const FdzJob<T>* jobs, const FdzLaunch& fl, const int* p
   constexpr int KT = kFdzKts[V];
   MLV_SET_LDS_ONCE((k_fdz_rows<T, KT>), fl.ldsrow);
   hipLaunchKernelGGL((k_fdz_rows<T, KT>), dim3(xcd_grid(nblocks)), dim3(64 * kFdzWaves), fl.ldsrow,
-                     s, jobs, pref, fl.njobs, phase, step, fl.zstride);
+                     s, jobs, const_cast<FdzJob<T>*>(jobs), pref, fl.njobs, phase, step,
+                     fl.zstride);
 }
 
 template <typename T>

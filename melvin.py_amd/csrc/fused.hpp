@@ -328,7 +328,15 @@ struct BatchLaunch {
   int* prefAp[2][2];
   // whether some job's z pass takes the split-pass plan under kPlanSplit
   int splitB;
+  // log2 of the x / z grid size shared by every job of the batch, -1 when
+  // the jobs differ: a batch of one grid size launches kernels compiled for
+  // that size alone (kFixedLogn), whose registers are their own rather than
+  // the largest over every size the generic kernels handle
+  int lognx_all, lognz_all;
 };
+
+// Grid sizes with kernels compiled for them alone: 2^9 .. 2^11.
+__host__ __device__ constexpr bool fixed_logn(int logn) { return logn >= 9 && logn <= 11; }
 
 // Select kernel A's block shape.
 inline void select_a_shape(BatchLaunch& bl, int shape) {
@@ -388,5 +396,12 @@ template <typename T>
 void prepare_zpass(const BatchLaunch& bl);
 template <typename T>
 void prepare_update(const BatchLaunch& bl);
+// The z pass of program v for a batch of one z size (fixed_logn), compiled
+// in its own translation unit (fused_b_fixed.hip).
+template <typename T>
+void launch_zpass_fixed(int v, const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int flags,
+                        const cx<T>* tw, hipStream_t s);
+template <typename T>
+void prepare_zpass_fixed(const BatchLaunch& bl);
 
 }  // namespace mlv

@@ -42,6 +42,10 @@ template <typename T, int V>
 static void launchB(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int flags,
                     const cx<T>* tw, hipStream_t s) {
   if (bl.nB[stage][V] == 0) return;
+  if (fixed_logn(bl.lognz_all)) {
+    launch_zpass_fixed<T>(V, jobs, bl, stage, flags, tw, s);
+    return;
+  }
   if (bl.planB == kPlanTwoSweep)
     launchB_plan<T, V, kPlanTwoSweep>(jobs, bl, stage, flags, tw, s);
   else
@@ -75,6 +79,7 @@ void prepare_zpass(const BatchLaunch& bl) {
   MLV_B_LDS(BV_MHD1)
   MLV_B_LDS(BV_MHD2)
 #undef MLV_B_LDS
+  if (fixed_logn(bl.lognz_all)) prepare_zpass_fixed<T>(bl);
 }
 
 template void launch_zpass<float>(const JobDev<float>*, const BatchLaunch&, int, int,

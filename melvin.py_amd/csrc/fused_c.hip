@@ -9,13 +9,27 @@ namespace mlv {
 template <typename T, int WIDE>
 __global__ void __launch_bounds__(kNT)
     __attribute__((amdgpu_waves_per_eu((WIDE || sizeof(T) == 8) ? 1 : 4)))
-    k_stepC(const JobDev<T>* __restrict__ jobs, const int* __restrict__ pref, int njobs,
-            int stage, int step, const cx<T>* __restrict__ tw) {
+    k_stepC(const JobDev<T>* __restrict__ jobs, JobDev<T>* clock, const int* __restrict__ pref,
+            int njobs, int stage, int step, const cx<T>* __restrict__ tw) {
   extern __shared__ char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int lb = xcd_logical(pref[njobs]);
   if (lb < 0) return;
-  roleC<T, WIDE>(jobs, pref, njobs, lb, stage, step, tw, lds);
+  roleC<T, WIDE>(jobs, clock, pref, njobs, lb, stage, step, tw, lds);
+}
+
+// A batch whose jobs all have 2^LOGN points along x (fixed_logn).
+template <typename T, int WIDE, int LOGN>
+__global__ void __launch_bounds__(kNT)
+    __attribute__((amdgpu_waves_per_eu((WIDE || sizeof(T) == 8) ? 1 : 4)))
+    k_stepC_fixed(const JobDev<T>* __restrict__ jobs, JobDev<T>* clock,
+                  const int* __restrict__ pref, int njobs, int stage, int step,
+                  const cx<T>* __restrict__ tw) {
+  extern __shared__ char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int lb = xcd_logical(pref[njobs]);
+  if (lb < 0) return;
+  roleC_fixed<T, WIDE, LOGN>(jobs, clock, pref, njobs, lb, stage, step, tw, lds);
 }
 
 template <typename T>
@@ -47,14 +61,29 @@ template <typename T>
 void launch_update(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int step,
                    const cx<T>* tw, hipStream_t s) {
   if (bl.nC[stage] == 0) return;
+  if (fixed_logn(bl.lognx_all)) {
+#define MLV_C_FIXED(W, L)                                                                     \
+  if (bl.wideC == W && bl.lognx_all == L) {                                                   \
+    MLV_SET_LDS_ONCE((k_stepC_fixed<T, W, L>), bl.ldsC);                                      \
+    hipLaunchKernelGGL((k_stepC_fixed<T, W, L>), dim3(xcd_grid(bl.nC[stage])), dim3(kNT),     \
+                       bl.ldsC, s, jobs, const_cast<JobDev<T>*>(jobs),                        \
+                       (const int*)bl.prefC[stage], bl.njobs, stage, step, tw);               \
+  }
+    MLV_C_FIXED(0, 9) MLV_C_FIXED(0, 10) MLV_C_FIXED(0, 11)
+    MLV_C_FIXED(1, 9) MLV_C_FIXED(1, 10) MLV_C_FIXED(1, 11)
+#undef MLV_C_FIXED
+    return;
+  }
   if (bl.wideC) {
     MLV_SET_LDS_ONCE((k_stepC<T, 1>), bl.ldsC);
     hipLaunchKernelGGL((k_stepC<T, 1>), dim3(xcd_grid(bl.nC[stage])), dim3(kNT), bl.ldsC, s,
-                       jobs, (const int*)bl.prefC[stage], bl.njobs, stage, step, tw);
+                       jobs, const_cast<JobDev<T>*>(jobs), (const int*)bl.prefC[stage], bl.njobs,
+                       stage, step, tw);
   } else {
     MLV_SET_LDS_ONCE((k_stepC<T, 0>), bl.ldsC);
     hipLaunchKernelGGL((k_stepC<T, 0>), dim3(xcd_grid(bl.nC[stage])), dim3(kNT), bl.ldsC, s,
-                       jobs, (const int*)bl.prefC[stage], bl.njobs, stage, step, tw);
+                       jobs, const_cast<JobDev<T>*>(jobs), (const int*)bl.prefC[stage], bl.njobs,
+                       stage, step, tw);
   }
 }
 
@@ -62,6 +91,11 @@ template <typename T>
 void prepare_update(const BatchLaunch& bl) {
   if (bl.ldsCp[1]) MLV_SET_LDS_ONCE((k_stepC<T, 1>), bl.ldsCp[1]);
   if (bl.ldsCp[0]) MLV_SET_LDS_ONCE((k_stepC<T, 0>), bl.ldsCp[0]);
+#define MLV_C_FIXED(W, L) \
+  if (bl.lognx_all == L && bl.ldsCp[W]) MLV_SET_LDS_ONCE((k_stepC_fixed<T, W, L>), bl.ldsCp[W]);
+  MLV_C_FIXED(0, 9) MLV_C_FIXED(0, 10) MLV_C_FIXED(0, 11)
+  MLV_C_FIXED(1, 9) MLV_C_FIXED(1, 10) MLV_C_FIXED(1, 11)
+#undef MLV_C_FIXED
 }
 
 template <typename T>

@@ -285,20 +285,21 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
 // Kernel C's work for logical block lb of a launch over the job table's
 // update blocks (prefix array pref).  The block of local index 0 of a job
 // advances the job's device-side t and loop count after its last stage
-// (Simulation.py:199-209); nothing else in the launch reads those fields.
+// (Simulation.py:199-209), through `clock` -- the same table, as a plain
+// pointer: the launch reads those two fields through neither pointer, so
+// the read-only, non-aliased `jobs` stays valid for every field it reads.
 template <typename T, int WIDE>
-__device__ __forceinline__ void roleC(const JobDev<T>* __restrict__ jobs, const int* __restrict__ pref,
-                                      int njobs, int lb, int stage, int step,
-                                      const cx<T>* __restrict__ tw, cx<T>* lds) {
+__device__ __forceinline__ void roleC(const JobDev<T>* __restrict__ jobs, JobDev<T>* clock,
+                                      const int* __restrict__ pref, int njobs, int lb, int stage,
+                                      int step, const cx<T>* __restrict__ tw, cx<T>* lds) {
   const int j = find_job(pref, njobs, lb);
   const JobDev<T>& J = jobs[j];
   const int local = lb - pref[j];
   const int par = (J.par0 + step) & 1;
   const int cur = ring(J.curr0, step, J.q);
   if (stage == J.nstage - 1 && local == 0 && threadIdx.x == 0) {
-    JobDev<T>* Jw = const_cast<JobDev<T>*>(jobs) + j;
-    Jw->t += J.dt;
-    Jw->loops += 1;
+    clock[j].t = clock[j].t + J.dt;
+    clock[j].loops = clock[j].loops + 1;
   }
   switch (J.lognx) {
 #define X(L)                                                            \
@@ -310,6 +311,24 @@ __device__ __forceinline__ void roleC(const JobDev<T>* __restrict__ jobs, const 
     default:
       break;
   }
+}
+
+// Kernel C's work for a batch whose jobs all have 2^LOGN points along x.
+template <typename T, int WIDE, int LOGN>
+__device__ __forceinline__ void roleC_fixed(const JobDev<T>* __restrict__ jobs, JobDev<T>* clock,
+                                            const int* __restrict__ pref, int njobs, int lb,
+                                            int stage, int step, const cx<T>* __restrict__ tw,
+                                            cx<T>* lds) {
+  const int j = find_job(pref, njobs, lb);
+  const JobDev<T>& J = jobs[j];
+  const int local = lb - pref[j];
+  const int par = (J.par0 + step) & 1;
+  const int cur = ring(J.curr0, step, J.q);
+  if (stage == J.nstage - 1 && local == 0 && threadIdx.x == 0) {
+    clock[j].t = clock[j].t + J.dt;
+    clock[j].loops = clock[j].loops + 1;
+  }
+  stepC_body<T, LOGN, WIDE>(J, J.st[stage], local, cur, par, tw, lds);
 }
 
 }  // namespace mlv

@@ -155,6 +155,20 @@ __device__ __forceinline__ void roleA(const JobDev<T>* __restrict__ jobs, const 
   }
 }
 
+// Kernel A's work for a batch whose jobs all have 2^LOGN points along x.
+template <typename T, int SHAPE, int LOGN>
+__device__ __forceinline__ void roleA_fixed(const JobDev<T>* __restrict__ jobs,
+                                            const int* __restrict__ pref, int njobs, int lb,
+                                            int stage, int step, const cx<T>* __restrict__ tw,
+                                            cx<T>* lds) {
+  const int j = find_job(pref, njobs, lb);
+  const JobDev<T>& J = jobs[j];
+  const int local = lb - pref[j];
+  const int par = (J.par0 + step) & 1;
+  if constexpr (xinv_cols<T, LOGN, fused_points<T>(LOGN, 0), SHAPE>() >= 1)
+    stepA_body<T, LOGN, SHAPE>(J, J.st[stage], local, par, tw, lds);
+}
+
 // --------------------------------------------------------------------------
 // Kernel M (single precision): the x mean of each channel an x-inverse op
 // of the stage writes, as a function of z -- the z c2r of the channel's

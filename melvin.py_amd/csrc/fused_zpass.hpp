@@ -404,4 +404,19 @@ __device__ __forceinline__ void roleB(const JobDev<T>* __restrict__ jobs, const 
   }
 }
 
+// Kernel B's work for a batch whose jobs all have 2^LOGN points along z.
+template <typename T, int V, int SPLIT, int PLAN, int LOGN>
+__device__ __forceinline__ void roleB_fixed(const JobDev<T>* __restrict__ jobs,
+                                            const int* __restrict__ pref, int njobs, int lb,
+                                            int stage, int flags, const cx<T>* __restrict__ tw,
+                                            cx<T>* lds) {
+  const int j = find_job(pref, njobs, lb);
+  const JobDev<T>& J = jobs[j];
+  const int local = lb - pref[j];
+  if constexpr (PLAN == kPlanSplit && bsplit_ok<T, LOGN, V>())
+    stepB_split<T, LOGN, V, SPLIT>(J, J.st[stage], local, flags, tw, lds);
+  else
+    stepB_body<T, LOGN, V, SPLIT>(J, J.st[stage], local, flags, tw, lds);
+}
+
 }  // namespace mlv
