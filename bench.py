@@ -454,8 +454,8 @@ def main():
         with HIP events on the launch stream."""
         import ctypes
 
-        tune = max(1, warmup // 4)
-        batch.step(max(0, warmup - 2 * tune))
+        tune = max(1, warmup // 6)
+        batch.step(max(0, warmup - 3 * tune))
         shapes.append(batch.autotune(steps=tune))
         torch.cuda.synchronize()
         t0 = time.perf_counter()

@@ -266,17 +266,21 @@ int mlv_batch_graph_step(mlv_batch* batch, int nsteps, void* stream);
 int mlv_batch_set_dt(mlv_batch* batch, int job, double dt, void* stream);
 /* Pick each fused kernel's variant for this batch by measurement (the
  * tuning the reference leaves to its array library's FFT plans,
- * SpectralTransformer.py:64-70): kernel A's block shape (0: 512 threads in
+ * SpectralTransformer.py:58,85,132,191): kernel A's block shape (0: 512 threads in
  * single precision, 256 in double; 1: half that, where every job's x
- * transform fits), kernel B's plan (0: split-pass -- inverse 16,16,R with
+ * transform fits; 2: shape 0's block holding two columns of every op of
+ * the stage side by side, so the ops derived from one field read it once --
+ * single precision, batches of one x size, 512 or 1024), kernel B's plan
+ * (0: split-pass -- inverse 16,16,R with
  * the products in its last pass, forward R,16,16 -- where a job's z
  * transform takes it, else the two-sweep plan; 1: the two-sweep plan
  * everywhere) and kernel C's tile shape (0 narrow, 1 wide: 32 points per
- * thread where it applies).  Runs nsteps real steps at every kernel's
- * option 0, at option 1, at 1 and at 0 (half of nsteps each) with events
- * around each kernel, so the steps advance the simulations as
- * mlv_batch_step would; a kernel's option figure is its faster segment per
- * step.  Keeps each kernel's faster option and remembers the choice for
+ * thread where it applies).  Runs nsteps real steps with every kernel at
+ * its first option, its second, .. up to the most options any kernel has,
+ * then back down (half of nsteps per segment; a kernel with fewer options
+ * stays at its first in the extra segments), with events around each
+ * kernel, so the steps advance the simulations as mlv_batch_step would; a
+ * kernel's option figure is its fastest segment per step.  Keeps each kernel's faster option and remembers the choice for
  * batches created later for the same set of job shapes (process-wide).
  * chosen (3 ints, or NULL) receives {shape A, plan B, shape C}.  A's shapes
  * and C's shapes give bit-identical results; B's plans agree to rounding.
@@ -284,11 +288,12 @@ int mlv_batch_set_dt(mlv_batch* batch, int job, double dt, void* stream);
 int mlv_batch_autotune(mlv_batch* batch, int nsteps, void* stream, int* chosen);
 /* The batch's kernel choices: plan[0..2] = {shape A, plan B, shape C} as
  * in mlv_batch_autotune, plan[3] = 1 if A's smaller shape is available,
- * plan[4] = 1 if some job's z pass takes the split-pass plan.  No
- * counterpart in the reference (its kernels are library calls). */
+ * plan[4] = 1 if some job's z pass takes the split-pass plan, plan[5] = 1
+ * if A's shape 2 is available: plan has 6 ints.  No counterpart in the
+ * reference (its kernels are library calls). */
 int mlv_batch_plan(const mlv_batch* batch, int* plan);
 /* Set the batch's kernel choices (values as in mlv_batch_autotune);
- * MLV_ERR_UNSUPPORTED for A's smaller shape where a job does not fit it.
+ * MLV_ERR_UNSUPPORTED for an x-inverse shape that does not apply.
  * Drops a captured graph.  No counterpart in the reference. */
 int mlv_batch_set_plan(mlv_batch* batch, int shape_xinv, int plan_zpass, int wide_update);
 /* nsteps loops with Simulation.end_loop's CFL ticker on the device

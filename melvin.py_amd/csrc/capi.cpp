@@ -926,7 +926,7 @@ int mlv_job_work_bytes(int problem, int nfield, int nx, int nz, int dtype, size_
 // the job set, ready for one upload (mlv_batch_create, mlv_batch_rebuild).
 struct BatchPlan {
   std::vector<char> host;  // JobDev table
-  std::vector<int> pA[2][2], pB[2][BV_COUNT], pC[2], pCn[2];  // pA[shape][stage]
+  std::vector<int> pA[3][2], pB[2][BV_COUNT], pC[2], pCn[2];  // pA[shape][stage]
   BatchLaunch bl{};        // block counts and LDS sizes; pointers set by upload_batch
   size_t ldsCn = 0;
   std::vector<int> nbB;
@@ -947,8 +947,7 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
   auto& pCn = P.pCn;
   size_t& ldsCn = P.ldsCn;
   for (int s = 0; s < 2; ++s) {
-    pA[0][s].assign(njobs + 1, 0);
-    pA[1][s].assign(njobs + 1, 0);
+    for (int a = 0; a < 3; ++a) pA[a][s].assign(njobs + 1, 0);
     pC[s].assign(njobs + 1, 0);
     pCn[s].assign(njobs + 1, 0);
     for (int v = 0; v < BV_COUNT; ++v) pB[s][v].assign(njobs + 1, 0);
@@ -956,7 +955,7 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
   BatchLaunch& bl = P.bl;
   std::memset(&bl, 0, sizeof(bl));
   bl.njobs = njobs;
-  bl.okA[0] = bl.okA[1] = 1;
+  bl.okA[0] = bl.okA[1] = bl.okA[2] = 1;
   bl.lognx_all = bl.lognz_all = 0;
   for (int j = 0; j < njobs; ++j) {
     int nstage, lognx, lognz, nm, nx;
@@ -1000,6 +999,13 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
     const int sla1 = f64 ? xinv_cols_host<double>(lognx, 1) : xinv_cols_host<float>(lognx, 1);
     const size_t ldsa1 = f64 ? xinv_lds_bytes<double>(lognx, 1) : xinv_lds_bytes<float>(lognx, 1);
     if (sla1 < 1) bl.okA[1] = 0;
+    // kernel A's mixed tiles (stepA_mixed): every stage's ops, kMixCols
+    // columns of each, in shape 0's block
+    {
+      int max_nA = 0;
+      for (int s = 0; s < nstage; ++s) max_nA = std::max(max_nA, nA[s]);
+      if (f64 || !xmix_fits<float>(lognx, max_nA)) bl.okA[2] = 0;
+    }
     const int sln = f64 ? fused_cols_host<double>(lognx, 0) : fused_cols_host<float>(lognx, 0);
     const int slw = f64 ? fused_cols_host<double>(lognx, 1) : fused_cols_host<float>(lognx, 1);
     const size_t ldsn = f64 ? fused_lds_bytes<double>(lognx, 0) : fused_lds_bytes<float>(lognx, 0);
@@ -1023,7 +1029,7 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
     }
     if (nstage > P.nstage) P.nstage = nstage;
     for (int s = 0; s < 2; ++s) {
-      int a = 0, a1 = 0, c = 0, cn = 0, bb = 0, bv = 0;
+      int a = 0, a1 = 0, a2 = 0, c = 0, cn = 0, bb = 0, bv = 0;
       if (s < nstage) {
         const int nin_v = bvar_nin(bvar[s]), neq_v = bvar_neq(bvar[s]);
         const int slz = zslots(bvar[s]);
@@ -1035,6 +1041,7 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
         const int rb = brows(nin_v, neq_v, slz);
         a = nA[s] * ((nm + sla - 1) / sla);
         a1 = sla1 >= 1 ? nA[s] * ((nm + sla1 - 1) / sla1) : 0;
+        a2 = (nm + kMixCols - 1) / kMixCols;
         if (f64 ? bsplit_fits<double>(lognz, bvar[s]) : bsplit_fits<float>(lognz, bvar[s]))
           bl.splitB = 1;
         c = neq[s] * ((nm + slw - 1) / slw);
@@ -1049,6 +1056,7 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
       }
       pA[0][s][j + 1] = pA[0][s][j] + a;
       pA[1][s][j + 1] = pA[1][s][j] + a1;
+      pA[2][s][j + 1] = pA[2][s][j] + a2;
       pC[s][j + 1] = pC[s][j] + c;
       pCn[s][j + 1] = pCn[s][j] + cn;
       for (int v = 0; v < BV_COUNT; ++v) pB[s][v][j + 1] = pB[s][v][j] + (v == bv ? bb : 0);
@@ -1056,6 +1064,7 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
     if (rc) break;
     bl.ldsAp[0] = std::max(bl.ldsAp[0], ldsa);
     if (sla1 >= 1) bl.ldsAp[1] = std::max(bl.ldsAp[1], ldsa1);
+    bl.ldsAp[2] = bl.ldsAp[0];
     // kernel M (single precision mean profiles): one LDS slot of nz
     if (!f64 && jobs[j].mean_split)
       bl.ldsM = std::max(bl.ldsM, (size_t)fft_ldsn(1 << lognz) * 8);
@@ -1075,6 +1084,8 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
         cus <= 0)
       cus = 256;
     bl.wideC = (dtype == MLV_F64 && pC[0][njobs] >= 4 * cus) ? 1 : 0;
+    // the mixed tiles run in the kernels compiled for one x size
+    if (!fixed_logn(bl.lognx_all)) bl.okA[2] = 0;
     // kernel A: the default block shape; kernel B: the split-pass plan
     bl.shapeA = 0;
     bl.planB = kPlanSplit;
@@ -1112,6 +1123,7 @@ static int upload_batch(BatchHost* b, BatchPlan& P, const mlv_job_desc* jobs, in
   for (int s = 0; s < 2; ++s) {
     arrays.push_back({&P.pA[0][s], &bl.prefAp[0][s]});
     arrays.push_back({&P.pA[1][s], &bl.prefAp[1][s]});
+    arrays.push_back({&P.pA[2][s], &bl.prefAp[2][s]});
     arrays.push_back({&P.pC[s], &bl.prefCp[1][s]});
     arrays.push_back({&P.pCn[s], &bl.prefCp[0][s]});
     for (int v = 0; v < BV_COUNT; ++v) arrays.push_back({&P.pB[s][v], &bl.prefB[s][v]});
@@ -1135,6 +1147,7 @@ static int upload_batch(BatchHost* b, BatchPlan& P, const mlv_job_desc* jobs, in
   for (int s = 0; s < 2; ++s) {
     bl.nAp[0][s] = P.pA[0][s][njobs];
     bl.nAp[1][s] = P.pA[1][s][njobs];
+    bl.nAp[2][s] = P.pA[2][s][njobs];
     bl.nCp[1][s] = P.pC[s][njobs];
     bl.nCp[0][s] = P.pCn[s][njobs];
     for (int v = 0; v < BV_COUNT; ++v) bl.nB[s][v] = P.pB[s][v][njobs];
@@ -1413,21 +1426,38 @@ int mlv_batch_autotune(mlv_batch* bp, int nsteps, void* stream, int* chosen) {
   // idle period, and a kernel's first launch, from deciding the choice.
   // The kernels read and write whole buffers in HBM between launches, so
   // one kernel's choice does not change another's time.
-  const int nopt[3] = {b->bl.okA[1] ? 2 : 1, b->bl.splitB ? 2 : 1, 2};
+  // the values each kernel can take
+  int opts[3][3], nopt[3] = {0, 0, 0};
+  for (int a = 0; a < 3; ++a)
+    if (b->bl.okA[a]) opts[0][nopt[0]++] = a;
+  opts[1][nopt[1]++] = kPlanSplit;
+  if (b->bl.splitB) opts[1][nopt[1]++] = kPlanTwoSweep;
+  opts[2][nopt[2]++] = 0;
+  opts[2][nopt[2]++] = 1;
+  const int maxopt = std::max(nopt[0], std::max(nopt[1], nopt[2]));
   const int h1 = nsteps >= 2 ? nsteps / 2 : nsteps, h2 = nsteps - h1;
-  const int seg_opt[4] = {0, 1, 1, 0};
-  const int seg_steps[4] = {h1, h1, h2, h2};
-  float best_ms[3][2] = {{INFINITY, INFINITY}, {INFINITY, INFINITY}, {INFINITY, INFINITY}};
+  // option indices 0, 1, .., maxopt - 1, then back down
+  int seg_opt[6], seg_steps[6];
+  const int nseg = 2 * maxopt;
+  for (int k = 0; k < maxopt; ++k) {
+    seg_opt[k] = k;
+    seg_steps[k] = h1;
+    seg_opt[nseg - 1 - k] = k;
+    seg_steps[nseg - 1 - k] = h2;
+  }
+  float best_ms[3][3];
+  for (auto& r : best_ms)
+    for (float& v : r) v = INFINITY;
   const int per_step = 4 * b->nstage;
   std::vector<hipEvent_t> ev((size_t)std::max(h1, h2) * per_step, nullptr);
   hipError_t e = hipSuccess;
   for (auto& x : ev)
     if (e == hipSuccess) e = hipEventCreate(&x);
-  for (int k = 0; k < 4 && e == hipSuccess; ++k) {
+  for (int k = 0; k < nseg && e == hipSuccess; ++k) {
     const int n = seg_steps[k];
     if (n == 0) continue;
     int opt[3];
-    for (int q = 0; q < 3; ++q) opt[q] = seg_opt[k] < nopt[q] ? seg_opt[k] : 0;
+    for (int q = 0; q < 3; ++q) opt[q] = opts[q][seg_opt[k] < nopt[q] ? seg_opt[k] : 0];
     apply_plan(b, opt[0], opt[1], opt[2]);
     for (int t = 0; t < n && e == hipSuccess; ++t) {
       const int step = (int)((b->steps + t) & 3);
@@ -1457,7 +1487,12 @@ int mlv_batch_autotune(mlv_batch* bp, int nsteps, void* stream, int* chosen) {
     if (x) (void)hipEventDestroy(x);
   if (e != hipSuccess) return hip_fail(e, "autotune");
   int best[3];
-  for (int q = 0; q < 3; ++q) best[q] = (nopt[q] > 1 && best_ms[q][1] < best_ms[q][0]) ? 1 : 0;
+  for (int q = 0; q < 3; ++q) {
+    int bi = 0;
+    for (int o = 1; o < nopt[q]; ++o)
+      if (best_ms[q][o] < best_ms[q][bi]) bi = o;
+    best[q] = opts[q][bi];
+  }
   apply_plan(b, best[0], best[1], best[2]);
   drop_graph(b);
   {
@@ -1477,17 +1512,22 @@ int mlv_batch_plan(const mlv_batch* bp, int* plan) {
   plan[2] = b->bl.wideC;
   plan[3] = b->bl.okA[1];
   plan[4] = b->bl.splitB;
+  plan[5] = b->bl.okA[2];
   return MLV_OK;
 }
 
 int mlv_batch_set_plan(mlv_batch* bp, int shape_xinv, int plan_zpass, int wide_update) {
   if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
   BatchHost* b = reinterpret_cast<BatchHost*>(bp);
-  if (shape_xinv < 0 || shape_xinv > 1 || plan_zpass < 0 || plan_zpass > 1 || wide_update < 0 ||
+  if (shape_xinv < 0 || shape_xinv > 2 || plan_zpass < 0 || plan_zpass > 1 || wide_update < 0 ||
       wide_update > 1)
-    return fail(MLV_ERR_ARG, "kernel choices are 0 or 1");
+    return fail(MLV_ERR_ARG, "kernel choices: x-inverse 0, 1 or 2, z pass and update 0 or 1");
   if (shape_xinv == 1 && !b->bl.okA[1])
     return fail(MLV_ERR_UNSUPPORTED, "some job's x transform does not fit the smaller x-inverse block");
+  if (shape_xinv == 2 && !b->bl.okA[2])
+    return fail(MLV_ERR_UNSUPPORTED,
+                "the mixed x-inverse tiles need single-precision jobs of one x size, 512 or 1024, "
+                "whose stages' ops fit two columns each in a block");
   apply_plan(b, shape_xinv, plan_zpass, wide_update);
   drop_graph(b);
   return MLV_OK;

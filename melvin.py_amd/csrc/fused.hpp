@@ -217,10 +217,12 @@ inline size_t fused_lds_bytes(int logn, int wide) {
 // only merged in L2 when their blocks ran side by side.  Double precision
 // takes 256 (4 columns of 16-byte values already fill a line, and 512 would
 // not fit the LDS at 4096 points).  Shape 1 halves the block (half the
-// columns per block, twice the blocks); the autotuner times both.
+// columns per block, twice the blocks); shape 2 keeps shape 0's block with
+// every op of the stage side by side (stepA_mixed); the autotuner times
+// them.
 template <typename T, int SHAPE = 0>
 __host__ __device__ constexpr int xinv_threads() {
-  return (sizeof(T) == 4 ? 512 : 256) >> SHAPE;
+  return (sizeof(T) == 4 ? 512 : 256) >> (SHAPE == 1 ? 1 : 0);
 }
 template <typename T, int LOGN, int P, int SHAPE = 0>
 __host__ __device__ constexpr int xinv_cols() {
@@ -230,13 +232,22 @@ __host__ __device__ constexpr int xinv_cols() {
 }
 template <typename T>
 inline int xinv_cols_host(int logn, int shape = 0) {
-  const int nt = (sizeof(T) == 4 ? 512 : 256) >> shape;
+  const int nt = (sizeof(T) == 4 ? 512 : 256) >> (shape == 1 ? 1 : 0);
   const int s = (nt * fused_points<T>(logn, 0)) >> logn;
   return s < nt ? s : nt;
 }
 template <typename T>
 inline size_t xinv_lds_bytes(int logn, int shape = 0) {
   return (size_t)xinv_cols_host<T>(logn, shape) * fft_ldsn(1 << logn) * sizeof(cx<T>);
+}
+// Columns per block of every op in kernel A's mixed tiles (shape 2), and
+// the host check that a job's stages fit them: single precision at 2^9 and
+// 2^10 points, kMixCols x (the most ops of any stage) <= the block's slots.
+constexpr int kMixCols = 2;
+template <typename T>
+inline bool xmix_fits(int logn, int max_nA) {
+  return sizeof(T) == 4 && (logn == 9 || logn == 10) &&
+         kMixCols * max_nA <= xinv_cols_host<T>(logn, 0);
 }
 
 // Rows per z-pass block: as many as fit the LDS slots, at most 8.
@@ -319,13 +330,14 @@ struct BatchLaunch {
   int nCp[2][2];
   size_t ldsCp[2];
   int* prefCp[2][2];
-  // both block shapes of kernel A (index = shape; okA[1] = 0 where some
-  // job's transform does not fit the smaller block); nA / ldsA / prefA above
-  // are the selected one
-  int okA[2];
-  int nAp[2][2];
-  size_t ldsAp[2];
-  int* prefAp[2][2];
+  // the block shapes of kernel A (index = shape; okA[s] = 0 where shape s
+  // does not apply to every job: the smaller block where a job's transform
+  // does not fit it, the mixed tiles outside xmix_fits or in a batch of
+  // several x sizes); nA / ldsA / prefA above are the selected one
+  int okA[3];
+  int nAp[3][2];
+  size_t ldsAp[3];
+  int* prefAp[3][2];
   // whether some job's z pass takes the split-pass plan under kPlanSplit
   int splitB;
   // log2 of the x / z grid size shared by every job of the batch, -1 when

@@ -54,6 +54,9 @@ void launch_xinv(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int st
   }
       MLV_A_FIXED(0, 9) MLV_A_FIXED(0, 10) MLV_A_FIXED(0, 11)
       MLV_A_FIXED(1, 9) MLV_A_FIXED(1, 10) MLV_A_FIXED(1, 11)
+      if constexpr (sizeof(T) == 4) {
+        MLV_A_FIXED(2, 9) MLV_A_FIXED(2, 10)
+      }
 #undef MLV_A_FIXED
     } else if (bl.shapeA == 1) {
       MLV_SET_LDS_ONCE((k_stepA<T, 1>), bl.ldsA);
@@ -79,10 +82,13 @@ void prepare_xinv(const BatchLaunch& bl) {
   if (bl.ldsAp[0]) MLV_SET_LDS_ONCE((k_stepA<T, 0>), bl.ldsAp[0]);
   if (bl.okA[1] && bl.ldsAp[1]) MLV_SET_LDS_ONCE((k_stepA<T, 1>), bl.ldsAp[1]);
 #define MLV_A_FIXED(SH, L)                                                   \
-  if (bl.lognx_all == L && bl.ldsAp[SH] && (SH == 0 || bl.okA[1]))           \
+  if (bl.lognx_all == L && bl.ldsAp[SH] && bl.okA[SH])                        \
     MLV_SET_LDS_ONCE((k_stepA_fixed<T, SH, L>), bl.ldsAp[SH]);
   MLV_A_FIXED(0, 9) MLV_A_FIXED(0, 10) MLV_A_FIXED(0, 11)
   MLV_A_FIXED(1, 9) MLV_A_FIXED(1, 10) MLV_A_FIXED(1, 11)
+  if constexpr (sizeof(T) == 4) {
+    MLV_A_FIXED(2, 9) MLV_A_FIXED(2, 10)
+  }
 #undef MLV_A_FIXED
   if constexpr (sizeof(T) == 4) {
     if (bl.ldsM) MLV_SET_LDS_ONCE(k_stepM<T>, bl.ldsM);

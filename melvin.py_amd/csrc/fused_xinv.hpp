@@ -130,6 +130,102 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
   }
 }
 
+// Kernel A with mixed tiles (block shape 2): a block holds kMixCols
+// columns of every op of the stage -- slot s is op s % nA of column
+// c0 + s / nA -- so the ops derived from one field (its identity, ux, uz)
+// read its columns once between them (from the cache after the first),
+// where one op per block reads each column once per op.  Each op's rows
+// are then stored in runs of kMixCols columns, and the neighbouring blocks
+// of the same XCD complete the lines.  Single precision at 2^9 and 2^10
+// points, 512 threads, when 2 nA fits the block's slots (xmix_ok).
+template <typename T, int LOGN>
+__host__ __device__ constexpr bool xmix_ok() {
+  return sizeof(T) == 4 && (LOGN == 9 || LOGN == 10) &&
+         fft_wave_local<(1 << LOGN), fused_points<T>(LOGN, 0)>() &&
+         xinv_cols<T, LOGN, fused_points<T>(LOGN, 0), 0>() *
+                 fft_tps<(1 << LOGN), fused_points<T>(LOGN, 0)>() ==
+             xinv_threads<T, 0>();
+}
+template <typename T, int LOGN>
+__device__ __forceinline__ void stepA_mixed(const JobDev<T>& J, const StageDev& S, int local,
+                                            int par, const cx<T>* __restrict__ tw, cx<T>* lds) {
+  constexpr int N = 1 << LOGN;
+  constexpr int P = fused_points<T>(LOGN, 0);
+  constexpr int NT = xinv_threads<T, 0>();
+  constexpr int TC = xinv_cols<T, LOGN, P, 0>();
+  constexpr int LDSN = fft_ldsn(N);
+  constexpr int TPS = fft_tps<N, P>();
+  constexpr int ITER = N / TPS;
+  constexpr int U = 8;
+  static_assert(xmix_ok<T, LOGN>() && ITER % U == 0, "mixed x-inverse tile shape");
+  const int nn = J.nn, nm = J.nm, rows = J.rows;
+  const int nA = S.nA;
+  const int c0 = local * kMixCols;
+  const int ncol = min(kMixCols, nm - c0);
+  const int nslot = ncol * nA;
+  // this thread's slot: op sl % nA of column c0 + sl / nA
+  const int sl = threadIdx.x / TPS;
+  const int xl = threadIdx.x % TPS;
+  const bool live = sl < nslot;
+  const XinvOp op = S.a[live ? sl % nA : 0];
+  const int m = c0 + sl / nA;
+  const cx<T>* __restrict__ src = J.state[op.from_new ? (par ^ 1) : par][op.src];
+  cx<T>* __restrict__ psi_out = (op.op == XOP_UX && op.src == 0 && !op.from_new) ? J.psi : nullptr;
+  const float ffx = (float)J.fx, ffz = (float)J.fz, ffx2 = (float)J.fx2, ffz2 = (float)J.fz2;
+#pragma unroll 1
+  for (int i0 = 0; i0 < ITER; i0 += U) {
+    cx<T> fv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int x = xl + (i0 + u) * TPS;
+      const int r = x <= nn ? x : (x >= N - nn ? x - N + rows : -1);
+      fv[u] = (live && r >= 0) ? src[(size_t)m * rows + r] : czero<T>();
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int x = xl + (i0 + u) * TPS;
+      cx<T> v = czero<T>();
+      const int r = x <= nn ? x : (x >= N - nn ? x - N + rows : -1);
+      if (live && r >= 0) {
+        const cx<T> f = fv[u];
+        if (op.op == XOP_ID) {
+          v = f;
+        } else {
+          const int n = r <= nn ? r : r - rows;
+          // the symbol and its reciprocal in FP32, as stepA_body
+          float lap = ffx2 * (float)(n * n) + ffz2 * (float)(m * m);
+          if (r == 0 && m == 0) lap = 1.0f;
+          const T rl = __builtin_amdgcn_rcpf(lap);
+          const cx<T> ps = mk<T>(-f.x * rl, -f.y * rl);
+          if (psi_out) psi_out[(size_t)m * rows + r] = ps;
+          if (op.op == XOP_UX) {
+            const T yz = ffz * (float)m;
+            v = mk<T>(yz * ps.y, -(yz * ps.x));
+          } else {
+            const T yx = ffx * (float)n;
+            v = mk<T>(-(yx * ps.y), yx * ps.x);
+          }
+        }
+        if (J.split && r == 0) v = czero<T>();
+      }
+      lds[sl * LDSN + fft_pad(x)] = v;
+    }
+  }
+  wave_sync();
+  fft_lds<T, LOGN, +1, NT, P>(lds, nslot, tw);
+  // per op: the row pairs' kMixCols columns, x % kXil fastest, then the
+  // column, so consecutive lanes store one contiguous run
+  for (int o = 0; o < nA; ++o) {
+    cx<T>* __restrict__ dst = J.work + (size_t)S.a[o].ch * N * nm;
+#pragma unroll 4
+    for (int e = threadIdx.x; e < kMixCols * N; e += NT) {
+      const int x2 = e % kXil, cc = (e / kXil) % kMixCols, x = (e / (kXil * kMixCols)) * kXil + x2;
+      if (cc < ncol)
+        dst[xrow_base(x, nm) + (size_t)(c0 + cc) * kXil] = lds[(cc * nA + o) * LDSN + fft_pad(x)];
+    }
+  }
+}
+
 // Kernel A's work for logical block lb of a launch over the job table's
 // x-inverse blocks (prefix array pref), in block shape SHAPE.  The host
 // selects a shape only when every job's transform fits its block
@@ -165,8 +261,11 @@ __device__ __forceinline__ void roleA_fixed(const JobDev<T>* __restrict__ jobs,
   const JobDev<T>& J = jobs[j];
   const int local = lb - pref[j];
   const int par = (J.par0 + step) & 1;
-  if constexpr (xinv_cols<T, LOGN, fused_points<T>(LOGN, 0), SHAPE>() >= 1)
+  if constexpr (SHAPE == 2) {
+    if constexpr (xmix_ok<T, LOGN>()) stepA_mixed<T, LOGN>(J, J.st[stage], local, par, tw, lds);
+  } else if constexpr (xinv_cols<T, LOGN, fused_points<T>(LOGN, 0), SHAPE>() >= 1) {
     stepA_body<T, LOGN, SHAPE>(J, J.st[stage], local, par, tw, lds);
+  }
 }
 
 // --------------------------------------------------------------------------

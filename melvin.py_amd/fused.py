@@ -351,7 +351,7 @@ class Batch(DeviceRun):
                   _lib.stream_handle(stream))
 
     # names of the kernel choices (mlv_batch_autotune)
-    _SHAPE_A = ("512 threads", "256 threads")
+    _SHAPE_A = ("512 threads", "256 threads", "512 threads, mixed ops")
     _PLAN_B = ("split-pass", "two-sweep")
     _SHAPE_C = ("narrow", "wide")
 
@@ -364,8 +364,9 @@ class Batch(DeviceRun):
 
     def autotune(self, steps=4, stream=None):
         """Pick each fused kernel's variant by timing ``steps`` real steps
-        with each option (mlv_batch_autotune); the jobs advance by 2 * steps.
-        Returns the chosen plan (see ``plan``)."""
+        with each option (mlv_batch_autotune); the jobs advance by 2 * steps
+        (3 * steps where the x-inverse has three shapes).  Returns the
+        chosen plan (see ``plan``)."""
         chosen = (ctypes.c_int * 3)()
         before = _lib.lib().mlv_batch_steps(self.handle)
         _lib.call("mlv_batch_autotune", self.handle, int(steps), _lib.stream_handle(stream), chosen)
@@ -376,17 +377,19 @@ class Batch(DeviceRun):
     def plan(self):
         """The kernel choices: x-inverse block shape, z-pass plan, update
         tile shape, and which alternatives this job set admits."""
-        p = (ctypes.c_int * 5)()
+        p = (ctypes.c_int * 6)()
         _lib.call("mlv_batch_plan", self.handle, p)
         double = self.jobs[0].dtype == "f64"
-        shape_a = ("256 threads", "128 threads") if double else self._SHAPE_A
+        shape_a = ("256 threads", "128 threads", "256 threads, mixed ops") if double else self._SHAPE_A
         return {"x_inverse": shape_a[p[0]], "z_pass": self._PLAN_B[p[1]],
                 "x_forward_update": self._SHAPE_C[p[2]],
                 "x_inverse_small_block_available": bool(p[3]),
-                "z_pass_split_available": bool(p[4])}
+                "z_pass_split_available": bool(p[4]),
+                "x_inverse_mixed_available": bool(p[5])}
 
     def set_plan(self, shape_xinv=0, plan_zpass=0, wide_update=0):
-        """Set the kernel choices (0/1 each, as in ``plan``)."""
+        """Set the kernel choices (x-inverse 0/1/2, z pass and update 0/1,
+        as in ``plan``)."""
         _lib.call("mlv_batch_set_plan", self.handle, int(shape_xinv), int(plan_zpass),
                   int(wide_update))
 

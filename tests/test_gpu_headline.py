@@ -101,12 +101,14 @@ def test_kh_every_kernel_plan_vs_oracle(mlv, n, dtype):
     st = _kh_oracle(p, start, 5)
     tol = 1e-12 if dtype == "f64" else 1e-5
     by_plan = {}
-    for a, zp, c in itertools.product((0, 1), (0, 1), (0, 1)):
+    for a, zp, c in itertools.product((0, 1, 2), (0, 1), (0, 1)):
         job = _kh_job(mlv, p, dtype, start)
         b = mlv.fused.Batch([job])
         avail = b.plan
         assert avail["z_pass_split_available"]
-        if a == 1 and not avail["x_inverse_small_block_available"]:
+        assert avail["x_inverse_mixed_available"] == (dtype == "f32")
+        if (a == 1 and not avail["x_inverse_small_block_available"]) or \
+                (a == 2 and not avail["x_inverse_mixed_available"]):
             b.close()
             continue
         print(f"[headline] KH {n}^2 {dtype} plan A{a} B{zp} C{c} x5 vs oracle", flush=True)
@@ -144,9 +146,8 @@ def test_kh_1024_fp32_vs_fp64_1000_steps(mlv):
         job = _kh_job(mlv, p, "f32", start, mean_split=split)
         other = _kh_job(mlv, p, "f32", _kh_start(p, seed=5), mean_split=split)
         b = mlv.fused.Batch([job, other])
-        tune = 3
-        b.autotune(steps=tune)
-        b.step(nsteps - 2 * tune)
+        b.autotune(steps=3)   # real steps of every option
+        b.step(nsteps - job.steps)
         w32, c32 = job.state()
         ew, ec = rel(w32, w64), rel(c32, c64)
         print(f"[headline]   omega {ew:.2e} scalar {ec:.2e}", flush=True)
