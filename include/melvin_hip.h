@@ -250,8 +250,11 @@ int mlv_batch_destroy(mlv_batch* batch);
  * tracker records; the jobs' clocks, ring slots and dt come from the
  * descriptors as at creation, and the batch's step count restarts at 0.
  * Synchronises the device first (queued steps may still read the old
- * tables) and drops a captured graph.  On an invalid descriptor the batch
- * is left unchanged.  Serves continuous batching (the reference has no
+ * tables) and drops a captured graph.  The kernel choices become those of
+ * a new batch of the same jobs (the autotuner's cached choice for the job
+ * set, else the defaults), and the kernels compiled for one grid size are
+ * used when the new set shares one.  On an invalid descriptor the batch is
+ * left unchanged.  Serves continuous batching (the reference has no
  * batches). */
 int mlv_batch_rebuild(mlv_batch* batch, const mlv_job_desc* jobs, int njobs);
 /* Run nsteps steps; write_phys = 1 also writes the physical fields of the

@@ -109,3 +109,37 @@ def test_serving_modes_agree(mlv):
     for x, y in zip(out[True], out[False]):
         for a, b in zip(x, y):
             np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_rebuild_between_size_specialised_and_generic_kernels(mlv, dtype):
+    """A batch of 512^2 jobs runs the kernels compiled for that size alone;
+    a 256^2 job joining switches the batch to the generic kernels, and its
+    leaving switches it back.  Every job equals the same steps run alone (a
+    512^2 job alone runs the specialised kernels throughout): both kernel
+    forms compute the same bits."""
+    S = mlv.serving
+
+    def jobs():
+        reqs = [S.Request(0, 0.0, "vortex", 512, 0, seed=21), S.Request(1, 0.0, "mhd", 512, 0, seed=22),
+                S.Request(2, 0.0, "vortex", 256, 0, seed=23)]
+        return [S.make_job(r, dtype) for r in reqs]
+
+    a, b, c = jobs()
+    batch = mlv.fused.Batch([a, b])
+    print(f"[serving] specialised/generic rebuilds {dtype}", flush=True)
+    batch.step(3)
+    batch.rebuild([a, c, b])        # mixed sizes: generic kernels
+    batch.step(4)
+    batch.rebuild([b, a])           # one size again: specialised kernels
+    batch.step(2)
+    batch.close()
+    ref = jobs()
+    for job, n in zip(ref, (9, 9, 4)):
+        bb = mlv.fused.Batch([job])
+        bb.step(n)
+        bb.close()
+    for got, want in zip((a, b, c), ref):
+        assert got.steps == want.steps
+        for x, y in zip(got.state(), want.state()):
+            np.testing.assert_array_equal(x, y)
