@@ -278,7 +278,8 @@ int mlv_batch_set_dt(mlv_batch* batch, int job, double dt, void* stream);
  * the products in its last pass, forward R,16,16 -- where a job's z
  * transform takes it, else the two-sweep plan; 1: the two-sweep plan
  * everywhere) and kernel C's tile shape (0 narrow, 1 wide: 32 points per
- * thread where it applies).  Runs nsteps real steps with every kernel at
+ * thread where it applies, 2 narrow in 512-thread blocks: single
+ * precision, batches of one x size in [512, 2048]).  Runs nsteps real steps with every kernel at
  * its first option, its second, .. up to the most options any kernel has,
  * then back down (half of nsteps per segment; a kernel with fewer options
  * stays at its first in the extra segments), with events around each
@@ -292,8 +293,8 @@ int mlv_batch_autotune(mlv_batch* batch, int nsteps, void* stream, int* chosen);
 /* The batch's kernel choices: plan[0..2] = {shape A, plan B, shape C} as
  * in mlv_batch_autotune, plan[3] = 1 if A's smaller shape is available,
  * plan[4] = 1 if some job's z pass takes the split-pass plan, plan[5] = 1
- * if A's shape 2 is available: plan has 6 ints.  No counterpart in the
- * reference (its kernels are library calls). */
+ * if A's shape 2 is available, plan[6] = 1 if C's shape 2 is: plan has 7
+ * ints.  No counterpart in the reference (its kernels are library calls). */
 int mlv_batch_plan(const mlv_batch* batch, int* plan);
 /* Set the batch's kernel choices (values as in mlv_batch_autotune);
  * MLV_ERR_UNSUPPORTED for an x-inverse shape that does not apply.

@@ -926,7 +926,7 @@ int mlv_job_work_bytes(int problem, int nfield, int nx, int nz, int dtype, size_
 // the job set, ready for one upload (mlv_batch_create, mlv_batch_rebuild).
 struct BatchPlan {
   std::vector<char> host;  // JobDev table
-  std::vector<int> pA[3][2], pB[2][BV_COUNT], pC[2], pCn[2];  // pA[shape][stage]
+  std::vector<int> pA[3][2], pB[2][BV_COUNT], pC[2], pCn[2], pCm[2];  // pA[shape][stage]
   BatchLaunch bl{};        // block counts and LDS sizes; pointers set by upload_batch
   size_t ldsCn = 0;
   std::vector<int> nbB;
@@ -950,12 +950,15 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
     for (int a = 0; a < 3; ++a) pA[a][s].assign(njobs + 1, 0);
     pC[s].assign(njobs + 1, 0);
     pCn[s].assign(njobs + 1, 0);
+    P.pCm[s].assign(njobs + 1, 0);
     for (int v = 0; v < BV_COUNT; ++v) pB[s][v].assign(njobs + 1, 0);
   }
   BatchLaunch& bl = P.bl;
   std::memset(&bl, 0, sizeof(bl));
   bl.njobs = njobs;
   bl.okA[0] = bl.okA[1] = bl.okA[2] = 1;
+  bl.okC[0] = bl.okC[1] = bl.okC[2] = 1;
+  size_t ldsCm = 0;
   bl.lognx_all = bl.lognz_all = 0;
   for (int j = 0; j < njobs; ++j) {
     int nstage, lognx, lognz, nm, nx;
@@ -1010,6 +1013,10 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
     const int slw = f64 ? fused_cols_host<double>(lognx, 1) : fused_cols_host<float>(lognx, 1);
     const size_t ldsn = f64 ? fused_lds_bytes<double>(lognx, 0) : fused_lds_bytes<float>(lognx, 0);
     const size_t ldsw = f64 ? fused_lds_bytes<double>(lognx, 1) : fused_lds_bytes<float>(lognx, 1);
+    // kernel C's 512-thread narrow shape (cshape_threads): single precision
+    const int slm = (cshape_threads(2) * 16) >> lognx;
+    const size_t ldsm = (size_t)slm * fft_ldsn(1 << lognx) * 8;
+    if (f64 || slm < 1 || slm > cshape_threads(2) || ldsm > lds_max) bl.okC[2] = 0;
     // z pass: points per thread by program (bpoints); the largest LDS any
     // of the job's programs takes is checked here
     auto zslots = [&](int v) {
@@ -1029,7 +1036,7 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
     }
     if (nstage > P.nstage) P.nstage = nstage;
     for (int s = 0; s < 2; ++s) {
-      int a = 0, a1 = 0, a2 = 0, c = 0, cn = 0, bb = 0, bv = 0;
+      int a = 0, a1 = 0, a2 = 0, c = 0, cn = 0, cm = 0, bb = 0, bv = 0;
       if (s < nstage) {
         const int nin_v = bvar_nin(bvar[s]), neq_v = bvar_neq(bvar[s]);
         const int slz = zslots(bvar[s]);
@@ -1046,6 +1053,7 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
           bl.splitB = 1;
         c = neq[s] * ((nm + slw - 1) / slw);
         cn = neq[s] * ((nm + sln - 1) / sln);
+        cm = slm >= 1 ? neq[s] * ((nm + slm - 1) / slm) : 0;
         bb = (nx + rb - 1) / rb;
         bv = bvar[s];
         bl.ldsB[bv] = std::max(bl.ldsB[bv], zlds(bv));
@@ -1059,11 +1067,13 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
       pA[2][s][j + 1] = pA[2][s][j] + a2;
       pC[s][j + 1] = pC[s][j] + c;
       pCn[s][j + 1] = pCn[s][j] + cn;
+      P.pCm[s][j + 1] = P.pCm[s][j] + cm;
       for (int v = 0; v < BV_COUNT; ++v) pB[s][v][j + 1] = pB[s][v][j] + (v == bv ? bb : 0);
     }
     if (rc) break;
     bl.ldsAp[0] = std::max(bl.ldsAp[0], ldsa);
     if (sla1 >= 1) bl.ldsAp[1] = std::max(bl.ldsAp[1], ldsa1);
+    ldsCm = std::max(ldsCm, ldsm);
     bl.ldsAp[2] = bl.ldsAp[0];
     // kernel M (single precision mean profiles): one LDS slot of nz
     if (!f64 && jobs[j].mean_split)
@@ -1084,8 +1094,12 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
         cus <= 0)
       cus = 256;
     bl.wideC = (dtype == MLV_F64 && pC[0][njobs] >= 4 * cus) ? 1 : 0;
-    // the mixed tiles run in the kernels compiled for one x size
-    if (!fixed_logn(bl.lognx_all)) bl.okA[2] = 0;
+    // the mixed tiles and the 512-thread update run in the kernels compiled
+    // for one x size; where it applies, the 512-thread update is the default
+    // (64 x 1024^2 FP32: 556 against 578 us per launch for the 256-thread
+    // narrow shape -- Px and Pz rows read in runs of 8 columns, not 4)
+    if (!fixed_logn(bl.lognx_all)) bl.okA[2] = bl.okC[2] = 0;
+    if (bl.okC[2]) bl.wideC = 2;
     // kernel A: the default block shape; kernel B: the split-pass plan
     bl.shapeA = 0;
     bl.planB = kPlanSplit;
@@ -1097,11 +1111,12 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
       if (it != g_tuned.end()) {
         bl.shapeA = bl.okA[it->second.shapeA] ? it->second.shapeA : 0;
         bl.planB = it->second.planB;
-        bl.wideC = it->second.wideC;
+        bl.wideC = bl.okC[it->second.wideC] ? it->second.wideC : bl.wideC;
       }
     }
     bl.ldsCp[1] = bl.ldsC;
     bl.ldsCp[0] = ldsCn;
+    bl.ldsCp[2] = bl.okC[2] ? ldsCm : 0;
   }
   return rc;
 }
@@ -1126,6 +1141,7 @@ static int upload_batch(BatchHost* b, BatchPlan& P, const mlv_job_desc* jobs, in
     arrays.push_back({&P.pA[2][s], &bl.prefAp[2][s]});
     arrays.push_back({&P.pC[s], &bl.prefCp[1][s]});
     arrays.push_back({&P.pCn[s], &bl.prefCp[0][s]});
+    arrays.push_back({&P.pCm[s], &bl.prefCp[2][s]});
     for (int v = 0; v < BV_COUNT; ++v) arrays.push_back({&P.pB[s][v], &bl.prefB[s][v]});
   }
   std::vector<size_t> aoff;
@@ -1150,6 +1166,7 @@ static int upload_batch(BatchHost* b, BatchPlan& P, const mlv_job_desc* jobs, in
     bl.nAp[2][s] = P.pA[2][s][njobs];
     bl.nCp[1][s] = P.pC[s][njobs];
     bl.nCp[0][s] = P.pCn[s][njobs];
+    bl.nCp[2][s] = P.pCm[s][njobs];
     for (int v = 0; v < BV_COUNT; ++v) bl.nB[s][v] = P.pB[s][v][njobs];
   }
   // diagnostics: the buffer not in use now, grown when needed
@@ -1432,8 +1449,8 @@ int mlv_batch_autotune(mlv_batch* bp, int nsteps, void* stream, int* chosen) {
     if (b->bl.okA[a]) opts[0][nopt[0]++] = a;
   opts[1][nopt[1]++] = kPlanSplit;
   if (b->bl.splitB) opts[1][nopt[1]++] = kPlanTwoSweep;
-  opts[2][nopt[2]++] = 0;
-  opts[2][nopt[2]++] = 1;
+  for (int c = 0; c < 3; ++c)
+    if (b->bl.okC[c]) opts[2][nopt[2]++] = c;
   const int maxopt = std::max(nopt[0], std::max(nopt[1], nopt[2]));
   const int h1 = nsteps >= 2 ? nsteps / 2 : nsteps, h2 = nsteps - h1;
   // option indices 0, 1, .., maxopt - 1, then back down
@@ -1513,6 +1530,7 @@ int mlv_batch_plan(const mlv_batch* bp, int* plan) {
   plan[3] = b->bl.okA[1];
   plan[4] = b->bl.splitB;
   plan[5] = b->bl.okA[2];
+  plan[6] = b->bl.okC[2];
   return MLV_OK;
 }
 
@@ -1520,8 +1538,11 @@ int mlv_batch_set_plan(mlv_batch* bp, int shape_xinv, int plan_zpass, int wide_u
   if (!bp) return fail(MLV_ERR_ARG, "batch is NULL");
   BatchHost* b = reinterpret_cast<BatchHost*>(bp);
   if (shape_xinv < 0 || shape_xinv > 2 || plan_zpass < 0 || plan_zpass > 1 || wide_update < 0 ||
-      wide_update > 1)
-    return fail(MLV_ERR_ARG, "kernel choices: x-inverse 0, 1 or 2, z pass and update 0 or 1");
+      wide_update > 2)
+    return fail(MLV_ERR_ARG, "kernel choices: x-inverse and update 0, 1 or 2, z pass 0 or 1");
+  if (wide_update == 2 && !b->bl.okC[2])
+    return fail(MLV_ERR_UNSUPPORTED,
+                "the 512-thread update shape needs single-precision jobs of one x size in [512, 2048]");
   if (shape_xinv == 1 && !b->bl.okA[1])
     return fail(MLV_ERR_UNSUPPORTED, "some job's x transform does not fit the smaller x-inverse block");
   if (shape_xinv == 2 && !b->bl.okA[2])

@@ -182,10 +182,16 @@ template <> struct BProg<BV_MHD2> {
 
 // Columns per block of the x-pass kernels: every LDS slot, but at most one
 // per thread, so that each thread keeps a single column (kNT % cols == 0).
-template <int LOGN, int P>
+template <int LOGN, int P, int NT = kNT>
 __host__ __device__ constexpr int fused_cols() {
-  return fft_slots<LOGN, kNT, P>() < kNT ? fft_slots<LOGN, kNT, P>() : kNT;
+  return fft_slots<LOGN, NT, P>() < NT ? fft_slots<LOGN, NT, P>() : NT;
 }
+// Kernel C's tile shapes (BatchLaunch::wideC): 0 narrow and 1 wide at kNT
+// threads (fused_points), 2 narrow at 512 threads -- twice the columns per
+// block, so each row of Px / Pz is read in runs of 8 (single precision,
+// batches of one x size, fixed_logn).
+__host__ __device__ constexpr int cshape_threads(int shape) { return shape == 2 ? 512 : kNT; }
+__host__ __device__ constexpr int cshape_wide(int shape) { return shape == 1 ? 1 : 0; }
 // Points per thread of the x passes.  wide = 1 is the column-pass policy
 // of fft_points_col (32 points from N = 1024 in single precision: more
 // columns per block); wide = 0 keeps 16 points up to N = 2048 (half the LDS
@@ -325,11 +331,13 @@ struct BatchLaunch {
   int* prefA[2];                       // device prefix arrays (njobs + 1)
   int* prefB[2][BV_COUNT];
   int* prefC[2];
-  // both tile shapes of kernel C (index = wide); wideC / nC / ldsC / prefC
+  // the tile shapes of kernel C (index = shape, cshape_*; okC[2] = 0 outside
+  // single-precision batches of one x size); wideC / nC / ldsC / prefC
   // above are the selected one
-  int nCp[2][2];
-  size_t ldsCp[2];
-  int* prefCp[2][2];
+  int okC[3];
+  int nCp[3][2];
+  size_t ldsCp[3];
+  int* prefCp[3][2];
   // the block shapes of kernel A (index = shape; okA[s] = 0 where shape s
   // does not apply to every job: the smaller block where a job's transform
   // does not fit it, the mixed tiles outside xmix_fits or in a batch of
@@ -361,12 +369,12 @@ inline void select_a_shape(BatchLaunch& bl, int shape) {
 }
 
 // Select kernel C's tile shape (wide = 1: fused_points' wide policy).
-inline void select_c_policy(BatchLaunch& bl, int wide) {
-  bl.wideC = wide;
-  bl.ldsC = bl.ldsCp[wide];
+inline void select_c_policy(BatchLaunch& bl, int shape) {
+  bl.wideC = shape;
+  bl.ldsC = bl.ldsCp[shape];
   for (int s = 0; s < 2; ++s) {
-    bl.nC[s] = bl.nCp[wide][s];
-    bl.prefC[s] = bl.prefCp[wide][s];
+    bl.nC[s] = bl.nCp[shape][s];
+    bl.prefC[s] = bl.prefCp[shape][s];
   }
 }
 

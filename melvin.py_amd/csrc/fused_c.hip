@@ -18,10 +18,11 @@ __global__ void __launch_bounds__(kNT)
   roleC<T, WIDE>(jobs, clock, pref, njobs, lb, stage, step, tw, lds);
 }
 
-// A batch whose jobs all have 2^LOGN points along x (fixed_logn).
-template <typename T, int WIDE, int LOGN>
-__global__ void __launch_bounds__(kNT)
-    __attribute__((amdgpu_waves_per_eu((WIDE || sizeof(T) == 8) ? 1 : 4)))
+// A batch whose jobs all have 2^LOGN points along x (fixed_logn), tile
+// shape SHAPE (cshape_*).
+template <typename T, int SHAPE, int LOGN>
+__global__ void __launch_bounds__(cshape_threads(SHAPE))
+    __attribute__((amdgpu_waves_per_eu((SHAPE == 1 || sizeof(T) == 8) ? 1 : 4)))
     k_stepC_fixed(const JobDev<T>* __restrict__ jobs, JobDev<T>* clock,
                   const int* __restrict__ pref, int njobs, int stage, int step,
                   const cx<T>* __restrict__ tw) {
@@ -29,7 +30,7 @@ __global__ void __launch_bounds__(kNT)
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int lb = xcd_logical(pref[njobs]);
   if (lb < 0) return;
-  roleC_fixed<T, WIDE, LOGN>(jobs, clock, pref, njobs, lb, stage, step, tw, lds);
+  roleC_fixed<T, SHAPE, LOGN>(jobs, clock, pref, njobs, lb, stage, step, tw, lds);
 }
 
 template <typename T>
@@ -65,12 +66,16 @@ void launch_update(const JobDev<T>* jobs, const BatchLaunch& bl, int stage, int 
 #define MLV_C_FIXED(W, L)                                                                     \
   if (bl.wideC == W && bl.lognx_all == L) {                                                   \
     MLV_SET_LDS_ONCE((k_stepC_fixed<T, W, L>), bl.ldsC);                                      \
-    hipLaunchKernelGGL((k_stepC_fixed<T, W, L>), dim3(xcd_grid(bl.nC[stage])), dim3(kNT),     \
+    hipLaunchKernelGGL((k_stepC_fixed<T, W, L>), dim3(xcd_grid(bl.nC[stage])),                \
+                       dim3(cshape_threads(W)),                                               \
                        bl.ldsC, s, jobs, const_cast<JobDev<T>*>(jobs),                        \
                        (const int*)bl.prefC[stage], bl.njobs, stage, step, tw);               \
   }
     MLV_C_FIXED(0, 9) MLV_C_FIXED(0, 10) MLV_C_FIXED(0, 11)
     MLV_C_FIXED(1, 9) MLV_C_FIXED(1, 10) MLV_C_FIXED(1, 11)
+    if constexpr (sizeof(T) == 4) {
+      MLV_C_FIXED(2, 9) MLV_C_FIXED(2, 10) MLV_C_FIXED(2, 11)
+    }
 #undef MLV_C_FIXED
     return;
   }
@@ -92,9 +97,13 @@ void prepare_update(const BatchLaunch& bl) {
   if (bl.ldsCp[1]) MLV_SET_LDS_ONCE((k_stepC<T, 1>), bl.ldsCp[1]);
   if (bl.ldsCp[0]) MLV_SET_LDS_ONCE((k_stepC<T, 0>), bl.ldsCp[0]);
 #define MLV_C_FIXED(W, L) \
-  if (bl.lognx_all == L && bl.ldsCp[W]) MLV_SET_LDS_ONCE((k_stepC_fixed<T, W, L>), bl.ldsCp[W]);
+  if (bl.lognx_all == L && bl.ldsCp[W] && bl.okC[W])                                       \
+    MLV_SET_LDS_ONCE((k_stepC_fixed<T, W, L>), bl.ldsCp[W]);
   MLV_C_FIXED(0, 9) MLV_C_FIXED(0, 10) MLV_C_FIXED(0, 11)
   MLV_C_FIXED(1, 9) MLV_C_FIXED(1, 10) MLV_C_FIXED(1, 11)
+  if constexpr (sizeof(T) == 4) {
+    MLV_C_FIXED(2, 9) MLV_C_FIXED(2, 10) MLV_C_FIXED(2, 11)
+  }
 #undef MLV_C_FIXED
 }
 

@@ -20,12 +20,12 @@ __device__ __forceinline__ T modk(const cx<T>* __restrict__ tw, int k, int logn,
   return (T(8) * s1 - s2) / (T)(6.0 * h);
 }
 
-template <typename T, int LOGN, int WIDE>
+template <typename T, int LOGN, int WIDE, int NT = kNT>
 __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S, int local, int cur,
                                            int par, const cx<T>* __restrict__ tw, cx<T>* lds) {
   constexpr int N = 1 << LOGN;
   constexpr int P = fused_points<T>(LOGN, WIDE);
-  constexpr int TC = fused_cols<LOGN, P>();
+  constexpr int TC = fused_cols<LOGN, P, NT>();
   constexpr int LDSN = fft_ldsn(N);
   const int nn = J.nn, nm = J.nm, rows = J.rows;
   const size_t chs = (size_t)N * nm;
@@ -43,14 +43,14 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   //   D = F_x[ FDx(Px) + i sz(kz) Pz ].
   const int order = J.fd_order;
   const T ihx = order == 2 ? (T)(1.0 / (2.0 * J.dx)) : (T)(1.0 / (3.0 * J.dx));
-  constexpr int ITER = (N * TC) / kNT;
+  constexpr int ITER = (N * TC) / NT;
   // loads in flight per thread: Px phase UP, Pz phase U (held beside the
   // stencil's registers)
   constexpr int UP = 8;
   constexpr int U = (sizeof(T) == 4 && !WIDE) ? 4 : 8;
   static_assert(ITER % U == 0 && ITER % UP == 0, "load batching");
   // sz depends only on the column; every thread keeps one column (e % TC
-  // is fixed because kNT is a multiple of TC)
+  // is fixed because NT is a multiple of TC)
   const int cme = threadIdx.x % TC;
   const T szc = modk<T>(tw, c0 + cme, J.lognz, order, J.dz);
   const bool cok = cme < ncol;
@@ -69,13 +69,13 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
       cx<T> pv[UP];
 #pragma unroll
       for (int u = 0; u < UP; ++u) {
-        const int x = (threadIdx.x + (i0 + u) * kNT) / TC;
+        const int x = (threadIdx.x + (i0 + u) * NT) / TC;
         pv[u] = cok ? pxc[(size_t)x * nm + col] : czero<T>();
         zv[i0 + u] = cok ? pzc[(size_t)x * nm + col] : czero<T>();
       }
 #pragma unroll
       for (int u = 0; u < UP; ++u) {
-        const int x = (threadIdx.x + (i0 + u) * kNT) / TC;
+        const int x = (threadIdx.x + (i0 + u) * NT) / TC;
         slot[fft_pad(x)] = pv[u];
       }
     }
@@ -84,7 +84,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
     cx<T> dv[ITER];
 #pragma unroll
     for (int i = 0; i < ITER; ++i) {
-      const int x = (threadIdx.x + i * kNT) / TC;
+      const int x = (threadIdx.x + i * NT) / TC;
       auto at = [&](int d) { return slot[fft_pad((x + d) & (N - 1))]; };
       cx<T> d;
       if (order == 2) {
@@ -100,7 +100,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < ITER; ++i) {
-      const int x = (threadIdx.x + i * kNT) / TC;
+      const int x = (threadIdx.x + i * NT) / TC;
       slot[fft_pad(x)] = dv[i];
     }
   } else {
@@ -112,7 +112,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
       cx<T> pa[U], pb[U], pz[U], pa2[U], pb2[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int x = (threadIdx.x + (i0 + u) * kNT) / TC;
+        const int x = (threadIdx.x + (i0 + u) * NT) / TC;
         if (cok) {
           pa[u] = pxc[(size_t)((x + 1) & (N - 1)) * nm + col];
           pb[u] = pxc[(size_t)((x - 1) & (N - 1)) * nm + col];
@@ -125,7 +125,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        const int x = (threadIdx.x + (i0 + u) * kNT) / TC;
+        const int x = (threadIdx.x + (i0 + u) * NT) / TC;
         cx<T> v = czero<T>();
         if (cok) {
           cx<T> d;
@@ -145,8 +145,8 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   // the epilogue gives column cc to wavefront cc when the block has as many
   // columns as wavefronts; with 64 threads per transform that wavefront
   // transformed the column itself
-  constexpr bool WL_OUT = fft_tps<N, P>() == 64 && TC == kNT / 64;
-  fft_lds<T, LOGN, -1, kNT, P, !WL_OUT>(lds, ncol, tw);
+  constexpr bool WL_OUT = fft_tps<N, P>() == 64 && TC == NT / 64;
+  fft_lds<T, LOGN, -1, NT, P, !WL_OUT>(lds, ncol, tw);
 
   const int q = J.q;
   const double dt = J.dt, alpha = J.alpha;
@@ -170,7 +170,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   const float fdt = (float)dt, fc2 = (float)c2;
   // one wavefront per column (WPC per column when the block has fewer
   // columns than wavefronts, splitting its row groups)
-  constexpr int NW = kNT / 64;
+  constexpr int NW = NT / 64;
   constexpr int WPC = TC >= NW ? 1 : NW / TC;
   const int wsub = wv % WPC;
 #pragma unroll 1
@@ -314,7 +314,7 @@ __device__ __forceinline__ void roleC(const JobDev<T>* __restrict__ jobs, JobDev
 }
 
 // Kernel C's work for a batch whose jobs all have 2^LOGN points along x.
-template <typename T, int WIDE, int LOGN>
+template <typename T, int SHAPE, int LOGN>
 __device__ __forceinline__ void roleC_fixed(const JobDev<T>* __restrict__ jobs, JobDev<T>* clock,
                                             const int* __restrict__ pref, int njobs, int lb,
                                             int stage, int step, const cx<T>* __restrict__ tw,
@@ -328,7 +328,8 @@ __device__ __forceinline__ void roleC_fixed(const JobDev<T>* __restrict__ jobs, 
     clock[j].t = clock[j].t + J.dt;
     clock[j].loops = clock[j].loops + 1;
   }
-  stepC_body<T, LOGN, WIDE>(J, J.st[stage], local, cur, par, tw, lds);
+  stepC_body<T, LOGN, cshape_wide(SHAPE), cshape_threads(SHAPE)>(J, J.st[stage], local, cur, par,
+                                                                 tw, lds);
 }
 
 }  // namespace mlv

@@ -353,7 +353,7 @@ class Batch(DeviceRun):
     # names of the kernel choices (mlv_batch_autotune)
     _SHAPE_A = ("512 threads", "256 threads", "512 threads, mixed ops")
     _PLAN_B = ("split-pass", "two-sweep")
-    _SHAPE_C = ("narrow", "wide")
+    _SHAPE_C = ("narrow", "wide", "narrow, 512 threads")
 
     def _advance(self, n):
         for j in self.jobs:
@@ -377,7 +377,7 @@ class Batch(DeviceRun):
     def plan(self):
         """The kernel choices: x-inverse block shape, z-pass plan, update
         tile shape, and which alternatives this job set admits."""
-        p = (ctypes.c_int * 6)()
+        p = (ctypes.c_int * 7)()
         _lib.call("mlv_batch_plan", self.handle, p)
         double = self.jobs[0].dtype == "f64"
         shape_a = ("256 threads", "128 threads", "256 threads, mixed ops") if double else self._SHAPE_A
@@ -385,10 +385,11 @@ class Batch(DeviceRun):
                 "x_forward_update": self._SHAPE_C[p[2]],
                 "x_inverse_small_block_available": bool(p[3]),
                 "z_pass_split_available": bool(p[4]),
-                "x_inverse_mixed_available": bool(p[5])}
+                "x_inverse_mixed_available": bool(p[5]),
+                "x_forward_update_512_available": bool(p[6])}
 
     def set_plan(self, shape_xinv=0, plan_zpass=0, wide_update=0):
-        """Set the kernel choices (x-inverse 0/1/2, z pass and update 0/1,
+        """Set the kernel choices (x-inverse and update 0/1/2, z pass 0/1,
         as in ``plan``)."""
         _lib.call("mlv_batch_set_plan", self.handle, int(shape_xinv), int(plan_zpass),
                   int(wide_update))

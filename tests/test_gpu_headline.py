@@ -101,14 +101,16 @@ def test_kh_every_kernel_plan_vs_oracle(mlv, n, dtype):
     st = _kh_oracle(p, start, 5)
     tol = 1e-12 if dtype == "f64" else 1e-5
     by_plan = {}
-    for a, zp, c in itertools.product((0, 1, 2), (0, 1), (0, 1)):
+    for a, zp, c in itertools.product((0, 1, 2), (0, 1), (0, 1, 2)):
         job = _kh_job(mlv, p, dtype, start)
         b = mlv.fused.Batch([job])
         avail = b.plan
         assert avail["z_pass_split_available"]
         assert avail["x_inverse_mixed_available"] == (dtype == "f32")
+        assert avail["x_forward_update_512_available"] == (dtype == "f32")
         if (a == 1 and not avail["x_inverse_small_block_available"]) or \
-                (a == 2 and not avail["x_inverse_mixed_available"]):
+                (a == 2 and not avail["x_inverse_mixed_available"]) or \
+                (c == 2 and not avail["x_forward_update_512_available"]):
             b.close()
             continue
         print(f"[headline] KH {n}^2 {dtype} plan A{a} B{zp} C{c} x5 vs oracle", flush=True)
