@@ -311,13 +311,29 @@ __host__ __device__ constexpr size_t xrow_base(int x, int nm) {
   return (size_t)(x / kXil) * (kXil * (size_t)nm) + (size_t)(x % kXil);
 }
 
+// Layout of the Px / Pz channels (kernel B writes them, kernel C reads
+// them): kz in groups of kPGroup, each group x-major, so element (x, k) is
+// at pidx(x, k, nx).  A column tile of kernel C (kPGroup columns, or half
+// that) is then one contiguous run, and a z-pass block, which holds a row
+// pair, stores each group of both rows as one whole 128-byte line (single
+// precision).  Row-major rows cost the update kernel 64-byte runs at a
+// stride of a row.
+constexpr int kPGroup = 8;
+__host__ __device__ constexpr size_t pidx(int x, int k, int nx) {
+  return (size_t)(k / kPGroup) * ((size_t)nx * kPGroup) + (size_t)x * kPGroup + (size_t)(k % kPGroup);
+}
+// Elements per workspace channel: (nx, nm), nm rounded up to whole groups.
+__host__ __device__ constexpr size_t work_chs(int nx, int nm) {
+  return (size_t)nx * (size_t)((nm + kPGroup - 1) / kPGroup * kPGroup);
+}
+
 // Rows of the mean-profile area (the most z-pass input channels).
 constexpr int kMeanCh = 6;
 // Workspace of a job, in complex elements: the z-pass inputs and the Px/Pz
-// outputs, (nin + 2 neq) x (nx, nm), then the mean profiles.
+// outputs, (nin + 2 neq) channels of work_chs, then the mean profiles.
 inline size_t fused_work_elems(int nin, int neq, int nx, int nz) {
   const int nm = (nz - 1) / 3;
-  return (size_t)(nin + 2 * neq) * nx * nm + (size_t)(kMeanCh / 2) * nz;
+  return (size_t)(nin + 2 * neq) * work_chs(nx, nm) + (size_t)(kMeanCh / 2) * nz;
 }
 
 struct BatchLaunch {

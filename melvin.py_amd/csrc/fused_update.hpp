@@ -1,7 +1,6 @@
 // Device body of the fused step's update kernel C (forward x pass, FD
-// operator, sources, Adams-Bashforth + semi-implicit update), shared by its
-// own kernel (fused_c.hip) and the multi-role kernel of the pipelined
-// schedule (fused_multi.hip).
+// operator, sources, Adams-Bashforth + semi-implicit update), launched by
+// fused_c.hip in its generic and size-specialised forms.
 #pragma once
 #include "fused.hpp"
 #include "diag_dev.hpp"
@@ -28,7 +27,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   constexpr int TC = fused_cols<LOGN, P, NT>();
   constexpr int LDSN = fft_ldsn(N);
   const int nn = J.nn, nm = J.nm, rows = J.rows;
-  const size_t chs = (size_t)N * nm;
+  const size_t chs = work_chs(N, nm);
   const size_t ssz = (size_t)rows * nm;
   const int ntile = (nm + TC - 1) / TC;
   const int eqi = local / ntile, tile = local % ntile;
@@ -54,7 +53,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   const int cme = threadIdx.x % TC;
   const T szc = modk<T>(tw, c0 + cme, J.lognz, order, J.dz);
   const bool cok = cme < ncol;
-  const size_t col = (size_t)(c0 + cme);
+  const int col = c0 + cme;
   cx<T>* __restrict__ slot = lds + cme * LDSN;
   // Single precision with at most 16 points per thread: Px is loaded once
   // into the slot and the stencil reads its neighbours from LDS (4 % faster
@@ -70,8 +69,8 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
 #pragma unroll
       for (int u = 0; u < UP; ++u) {
         const int x = (threadIdx.x + (i0 + u) * NT) / TC;
-        pv[u] = cok ? pxc[(size_t)x * nm + col] : czero<T>();
-        zv[i0 + u] = cok ? pzc[(size_t)x * nm + col] : czero<T>();
+        pv[u] = cok ? pxc[pidx(x, col, N)] : czero<T>();
+        zv[i0 + u] = cok ? pzc[pidx(x, col, N)] : czero<T>();
       }
 #pragma unroll
       for (int u = 0; u < UP; ++u) {
@@ -114,12 +113,12 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
       for (int u = 0; u < U; ++u) {
         const int x = (threadIdx.x + (i0 + u) * NT) / TC;
         if (cok) {
-          pa[u] = pxc[(size_t)((x + 1) & (N - 1)) * nm + col];
-          pb[u] = pxc[(size_t)((x - 1) & (N - 1)) * nm + col];
-          pz[u] = pzc[(size_t)x * nm + col];
+          pa[u] = pxc[pidx((x + 1) & (N - 1), col, N)];
+          pb[u] = pxc[pidx((x - 1) & (N - 1), col, N)];
+          pz[u] = pzc[pidx(x, col, N)];
           if (order != 2) {
-            pa2[u] = pxc[(size_t)((x + 2) & (N - 1)) * nm + col];
-            pb2[u] = pxc[(size_t)((x - 2) & (N - 1)) * nm + col];
+            pa2[u] = pxc[pidx((x + 2) & (N - 1), col, N)];
+            pb2[u] = pxc[pidx((x - 2) & (N - 1), col, N)];
           }
         }
       }

@@ -28,7 +28,7 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
   const int c0 = tile * TC;
   const int ncol = min(TC, nm - c0);
   const cx<T>* __restrict__ src = J.state[op.from_new ? (par ^ 1) : par][op.src];
-  cx<T>* __restrict__ dst = J.work + (size_t)op.ch * N * nm;
+  cx<T>* __restrict__ dst = J.work + (size_t)op.ch * work_chs(N, nm);
   cx<T>* __restrict__ psi_out = (op.op == XOP_UX && op.src == 0 && !op.from_new) ? J.psi : nullptr;
   const double fx = J.fx, fz = J.fz, fx2 = J.fx2, fz2 = J.fz2;
   const float ffx = (float)fx, ffz = (float)fz, ffx2 = (float)fx2, ffz2 = (float)fz2;
@@ -216,7 +216,7 @@ __device__ __forceinline__ void stepA_mixed(const JobDev<T>& J, const StageDev& 
   // per op: the row pairs' kMixCols columns, x % kXil fastest, then the
   // column, so consecutive lanes store one contiguous run
   for (int o = 0; o < nA; ++o) {
-    cx<T>* __restrict__ dst = J.work + (size_t)S.a[o].ch * N * nm;
+    cx<T>* __restrict__ dst = J.work + (size_t)S.a[o].ch * work_chs(N, nm);
 #pragma unroll 4
     for (int e = threadIdx.x; e < kMixCols * N; e += NT) {
       const int x2 = e % kXil, cc = (e / kXil) % kMixCols, x = (e / (kXil * kMixCols)) * kXil + x2;

@@ -1,7 +1,6 @@
 // Device bodies of the fused step's z-pass kernel B (c2r of the channel
-// rows, physical products, r2c of the products), shared by its own kernel
-// (fused_b.hip) and the multi-role kernel of the pipelined schedule
-// (fused_multi.hip).
+// rows, physical products, r2c of the products), launched by fused_b.hip
+// (generic) and fused_b_fixed.hip (one z size).
 #pragma once
 #include "fused.hpp"
 #include "diag_dev.hpp"
@@ -179,29 +178,33 @@ __device__ __forceinline__ void bpoint(const JobDev<T>& J, const StageDev& S, bo
 }
 
 // Output phase: the kz < nm coefficients of each product row, split from
-// the packed (Px + i Pz) spectrum, into the Px / Pz channels (row-major).
-// One wavefront per product slot (WPO per slot when there are fewer).
+// the packed (Px + i Pz) spectrum, into the Px / Pz channels (pidx layout).
+// A work item is (equation, row pair, four kz groups); its lanes take row
+// (lane >> 3) & 1 of the pair, group (lane >> 4) of the four and kz % 8 =
+// lane & 7, so each 16 lanes store one group of both rows -- one whole line
+// per channel.  Every product slot must be complete for the whole block.
 template <typename T, typename Sh>
-__device__ __forceinline__ void bstore(cx<T>* __restrict__ work, size_t chs, int nm, int x0,
-                                       int nout, const cx<T>* lds) {
-  constexpr int N = Sh::N, NIN = Sh::NIN, NEQ = Sh::NEQ, KIT = Sh::KIT, WPO = Sh::WPO;
-  constexpr int NW = Sh::NW;
+__device__ __forceinline__ void bstore(cx<T>* __restrict__ work, size_t chs, int nm, int nx,
+                                       int x0, int nrows, const cx<T>* lds) {
+  constexpr int N = Sh::N, NIN = Sh::NIN, NEQ = Sh::NEQ, RB = Sh::RB, NW = Sh::NW;
+  static_assert(kPGroup == 8, "lane mapping: 8 kz per group");
+  constexpr int NPAIR = (RB + 1) / 2;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int osub = wv % WPO;
-  for (int o = wv / WPO; o < nout; o += NW / WPO) {
-    const int r = o / NEQ, eq = o % NEQ;
-    const cx<T>* z = lds + o * Sh::LDSN;
-    cx<T>* __restrict__ ox = work + (size_t)pchan(NIN, eq, 0) * chs + (size_t)(x0 + r) * nm;
-    cx<T>* __restrict__ oz = work + (size_t)pchan(NIN, eq, 1) * chs + (size_t)(x0 + r) * nm;
-#pragma unroll 6
-    for (int i = osub; i < KIT; i += WPO) {
-      const int k = lane + 64 * i;
-      if (k < nm) {
-        const cx<T> y = z[fft_pad(k)];
-        const cx<T> ym = z[fft_pad((N - k) & (N - 1))];
-        ox[k] = mk<T>(T(0.5) * (y.x + ym.x), T(0.5) * (y.y - ym.y));
-        oz[k] = mk<T>(T(0.5) * (y.y + ym.y), T(0.5) * (ym.x - y.x));
-      }
+  const int r2 = (lane >> 3) & 1, gsub = lane >> 4, kk = lane & 7;
+  const int nquad = (nm + 4 * kPGroup - 1) / (4 * kPGroup);
+  const int nitem = NEQ * NPAIR * nquad;
+  for (int it = wv; it < nitem; it += NW) {
+    const int eq = it % NEQ, rest = it / NEQ;
+    const int pr = rest % NPAIR, quad = rest / NPAIR;
+    const int r = 2 * pr + r2;
+    const int k = (quad * 4 + gsub) * kPGroup + kk;
+    if (r < nrows && k < nm) {
+      const cx<T>* z = lds + (r * NEQ + eq) * Sh::LDSN;
+      const cx<T> y = z[fft_pad(k)];
+      const cx<T> ym = z[fft_pad((N - k) & (N - 1))];
+      const size_t at = pidx(x0 + r, k, nx);
+      work[(size_t)pchan(NIN, eq, 0) * chs + at] = mk<T>(T(0.5) * (y.x + ym.x), T(0.5) * (y.y - ym.y));
+      work[(size_t)pchan(NIN, eq, 1) * chs + at] = mk<T>(T(0.5) * (y.y + ym.y), T(0.5) * (ym.x - y.x));
     }
   }
 }
@@ -219,7 +222,7 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     return;  // rejected on the host
   } else {
     const int nx = J.nx, nm = J.nm;
-    const size_t chs = (size_t)nx * nm;
+    const size_t chs = work_chs(nx, nm);
     const int x0 = local * RB;
     const int nrows = min(RB, nx - x0);
     const int nseq = nrows * NIN;
@@ -259,11 +262,8 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     if (diag) block_diag<kNT>((double)d.mux, (double)d.muz, d.ke, d.ens, d.flux,
                               J.diag + (size_t)kDiagPart * local);
     __syncthreads();
-    // when wavefront o transformed product slot o itself the outputs need no
-    // workgroup barrier after the transform
-    constexpr bool WL_OUT = fft_tps<N, P>() == 64 && Sh::NO == Sh::NW;
-    fft_lds<T, LOGN, -1, kNT, P, !WL_OUT>(lds, nrows * NEQ, tw);
-    bstore<T, Sh>(J.work, chs, nm, x0, nrows * NEQ, lds);
+    fft_lds<T, LOGN, -1, kNT, P, true>(lds, nrows * NEQ, tw);
+    bstore<T, Sh>(J.work, chs, nm, nx, x0, nrows, lds);
   }
 }
 
@@ -288,7 +288,7 @@ __device__ __forceinline__ void stepB_split(const JobDev<T>& J, const StageDev& 
   static_assert(LOGR >= 1 && LOGR <= 3 && kNT == NSL, "split pass shape");
   static_assert(fft_pass_logr(LOGN, 8) == LOGR, "standard plan ends with radix N/256");
   const int nx = J.nx, nm = J.nm;
-  const size_t chs = (size_t)nx * nm;
+  const size_t chs = work_chs(nx, nm);
   const int x0 = local * RB;
   const int nrows = min(RB, nx - x0);
   const int nseq = nrows * NIN;
@@ -369,9 +369,8 @@ __device__ __forceinline__ void stepB_split(const JobDev<T>& J, const StageDev& 
   // the remaining forward passes: radix 16 at Ns = R and Ns = 16 R
   fft_pass<T, LOGN, 4, LOGR, -1, kNT, P, true>(lds, nrows * NEQ, tw);
   fft_pass<T, LOGN, 4, LOGR + 4, -1, kNT, P, true>(lds, nrows * NEQ, tw);
-  constexpr bool WL_OUT = fft_tps<N, P>() == 64 && Sh::NO == Sh::NW;
-  if constexpr (!WL_OUT) __syncthreads();
-  bstore<T, Sh>(J.work, chs, nm, x0, nrows * NEQ, lds);
+  __syncthreads();
+  bstore<T, Sh>(J.work, chs, nm, nx, x0, nrows, lds);
 }
 
 template <typename T, int LOGN, int V>

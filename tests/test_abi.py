@@ -86,9 +86,10 @@ def test_argument_errors_are_reported():
     n = ctypes.c_size_t()
     assert lib.mlv_job_work_bytes(9, 1, 32, 32, 0, ctypes.byref(n)) == L.MLV_ERR_ARG
     assert lib.mlv_job_work_bytes(L.PROB_VORTICITY, 1, 32, 32, 0, ctypes.byref(n)) == 0
-    # TGV channels: w, ux, uz + Px, Pz -> 5 * nx * nm complex64, then the
-    # x-mean profiles of six channels (3 * nz complex64)
-    assert n.value == 5 * 32 * 10 * 8 + 3 * 32 * 8
+    # TGV channels: w, ux, uz + Px, Pz -> 5 channels of nx x (nm rounded up
+    # to whole groups of 8 kz) complex64, then the x-mean profiles of six
+    # channels (3 * nz complex64)
+    assert n.value == 5 * 32 * 16 * 8 + 3 * 32 * 8
 
 
 def test_fdz_job_desc_layout_matches_c(tmp_path):
