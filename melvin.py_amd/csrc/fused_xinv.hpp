@@ -41,7 +41,9 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
   // needed before the transform.  Otherwise rows are fastest over the
   // whole block.
   constexpr int ITER = (N * TC) / NT;
-  constexpr int U = 8;
+  // every load of the thread in flight at once where the registers allow
+  // (single precision, up to 16 points: one memory round trip)
+  constexpr int U = (sizeof(T) == 4 && ITER <= 16) ? ITER : 8;
   static_assert(ITER % U == 0, "load batching");
   constexpr int TPS = fft_tps<N, P>();
   constexpr bool WL = fft_wave_local<N, P>() && TC * TPS == NT;
