@@ -16,6 +16,21 @@ template <> struct CxOf<float> { using type = float2; };
 template <> struct CxOf<double> { using type = double2; };
 template <typename T> using cx = typename CxOf<T>::type;
 
+// A device pointer read out of a job table is generic to the compiler, so its
+// loads and stores become flat instructions, which also count against the
+// LDS counter and make every LDS wait wait for them.  glb() states that the
+// pointer is global memory; code inlined after it then issues global_* ops.
+// (The host pass of the same source sees plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+#else
+template <typename T>
+using gptr = T*;
+#endif
+template <typename T>
+__device__ __forceinline__ gptr<T> glb(T* p) { return (gptr<T>)p; }
+
 template <typename T>
 __device__ __forceinline__ cx<T> mk(T re, T im) {
   cx<T> r;

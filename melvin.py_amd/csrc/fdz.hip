@@ -159,7 +159,7 @@ __device__ __forceinline__ double2 d2dz2_row(const double2* __restrict__ b, int 
 // tridiagonal solves) and vrow (the state v_n the update starts from).
 // --------------------------------------------------------------------------
 template <typename T, int KT>
-__device__ void fdz_row(const FdzJob<T>& J, int stage, int n, int f, int step,
+__device__ __forceinline__ void fdz_row(const FdzJob<T>& J, int stage, int n, int f, int step,
                         double2* __restrict__ buf, double2* __restrict__ vrow) {
   const int lane = threadIdx.x & 63;
   const int Z = J.nz, K = J.K;
@@ -172,10 +172,10 @@ __device__ void fdz_row(const FdzJob<T>& J, int stage, int n, int f, int step,
   if (stage != FDZ_PROLOGUE) {
     const int q = J.q;
     const int cur = ring(J.curr0, step, q);
-    const cx<T>* __restrict__ qx = J.qw[2 * f] + row;
-    const cx<T>* __restrict__ qz = J.qw[2 * f + 1] + row;
-    const cx<T>* __restrict__ vin = J.state[par][f] + row;
-    const cx<T>* __restrict__ cin = J.comp[f] ? J.comp[f] + row : nullptr;
+    const cx<T>* __restrict__ qx = glb(J.qw[2 * f]) + row;
+    const cx<T>* __restrict__ qz = glb(J.qw[2 * f + 1]) + row;
+    const cx<T>* __restrict__ vin = glb(J.state[par][f]) + row;
+    const cx<T>* __restrict__ cin = J.comp[f] ? glb(J.comp[f]) + row : nullptr;
 #pragma unroll
     for (int i = 0; i < KT; ++i) {
       const int j = lane + 64 * i;
@@ -198,7 +198,7 @@ __device__ void fdz_row(const FdzJob<T>& J, int stage, int n, int f, int step,
     }
     // -Pr Ra i (2 pi n / lx) T  (rayleigh_benard_convection.py:101-104)
     const double buoy = f == 0 ? J.Pr * J.Ra * (J.fx * (double)n) : 0.0;
-    const cx<T>* __restrict__ tsrc = (stage == FDZ_CORR ? J.star[1] : J.state[par][1]) + row;
+    const cx<T>* __restrict__ tsrc = glb(stage == FDZ_CORR ? J.star[1] : J.state[par][1]) + row;
     const double coeff = f == 0 ? J.Pr : 1.0;
     const double k2 = J.fx2 * (double)n * (double)n;  // -(2 pi n / lx)^2
     const double idz2 = 1.0 / (dz * dz);
@@ -206,7 +206,7 @@ __device__ void fdz_row(const FdzJob<T>& J, int stage, int n, int f, int step,
     const int ws = stage == FDZ_CORR ? (cur + 1) % q : cur;  // ring slot written
     const int s1 = (ws + q - 1) % q, s2 = (ws + q - 2) % q, s3 = (ws + q - 3) % q;
     const size_t ssz = (size_t)J.nn * Z;
-    cx<T>* __restrict__ hist = J.hist[f];
+    cx<T>* __restrict__ hist = glb(J.hist[f]);
     const double dt = J.dt;
     double2 inc[KT];
 #pragma unroll
@@ -272,13 +272,13 @@ __device__ void fdz_row(const FdzJob<T>& J, int stage, int n, int f, int step,
       // increment form of the theta scheme:
       //   (I - a dt c L) delta = dt c L v + E,  v_new = v + delta
       wave_sync();
-      wave_tridiag<KT>(buf, J.fac[1 + f] + (size_t)n * K * 64, J.lo[1 + f], Z, K, lane);
+      wave_tridiag<KT>(buf, glb(J.fac[1 + f]) + (size_t)n * K * 64, J.lo[1 + f], Z, K, lane);
     }
     wave_sync();
     // new state and wall writes (rayleigh_benard_convection.py:113-142);
     // buf[j] becomes -omega_new for the Poisson solve (same lane, same j)
-    cx<T>* __restrict__ out = (stage == FDZ_PRED ? J.star[f] : J.state[par ^ 1][f]) + row;
-    cx<T>* __restrict__ cout = (stage != FDZ_PRED && J.comp[f]) ? J.comp[f] + row : nullptr;
+    cx<T>* __restrict__ out = glb(stage == FDZ_PRED ? J.star[f] : J.state[par ^ 1][f]) + row;
+    cx<T>* __restrict__ cout = (stage != FDZ_PRED && J.comp[f]) ? glb(J.comp[f]) + row : nullptr;
 #pragma unroll
     for (int i = 0; i < KT; ++i) {
       const int j = lane + 64 * i;
@@ -305,8 +305,8 @@ __device__ void fdz_row(const FdzJob<T>& J, int stage, int n, int f, int step,
       }
     }
   } else {
-    const cx<T>* __restrict__ vin = J.state[par][0] + row;
-    const cx<T>* __restrict__ cin = J.comp[0] ? J.comp[0] + row : nullptr;
+    const cx<T>* __restrict__ vin = glb(J.state[par][0]) + row;
+    const cx<T>* __restrict__ cin = J.comp[0] ? glb(J.comp[0]) + row : nullptr;
 #pragma unroll
     for (int i = 0; i < KT; ++i) {
       const int j = lane + 64 * i;
@@ -325,12 +325,12 @@ __device__ void fdz_row(const FdzJob<T>& J, int stage, int n, int f, int step,
   // psi = solve(-omega) (utility.py:65, LaplacianSolver.py:70-78),
   // ux_s = -pddz(psi), uz_s = i kx psi (utility.py:66-79)
   wave_sync();
-  wave_tridiag<KT>(buf, J.fac[0] + (size_t)n * K * 64, J.lo[0], Z, K, lane);
+  wave_tridiag<KT>(buf, glb(J.fac[0]) + (size_t)n * K * 64, J.lo[0], Z, K, lane);
   wave_sync();
   const int vs = stage == FDZ_PRED ? 2 : (stage == FDZ_PROLOGUE ? par : par ^ 1);
-  cx<T>* __restrict__ ux = J.vel[vs][0] + row;
-  cx<T>* __restrict__ uz = J.vel[vs][1] + row;
-  cx<T>* __restrict__ ps = vs < 2 && J.psi[vs] ? J.psi[vs] + row : nullptr;
+  cx<T>* __restrict__ ux = glb(J.vel[vs][0]) + row;
+  cx<T>* __restrict__ uz = glb(J.vel[vs][1]) + row;
+  cx<T>* __restrict__ ps = vs < 2 && J.psi[vs] ? glb(J.psi[vs]) + row : nullptr;
   const double kx = J.fx * (double)n;
 #pragma unroll
   for (int i = 0; i < KT; ++i) {
@@ -423,9 +423,9 @@ __device__ __forceinline__ void fdz_cols_body(const FdzJob<T>& J, int local, int
     const int npair = (ncol + 1) / 2;
     const int vs = corr ? 2 : par;
     // field f of column pair p lives in slot 4 p + f: 0 ux, 1 uz, 2 omega, 3 T
-    const cx<T>* __restrict__ src[4] = {J.vel[vs][0], J.vel[vs][1],
-                                        corr ? J.star[0] : J.state[par][0],
-                                        corr ? J.star[1] : J.state[par][1]};
+    const gptr<const cx<T>> src[4] = {glb(J.vel[vs][0]), glb(J.vel[vs][1]),
+                                      glb(corr ? J.star[0] : J.state[par][0]),
+                                      glb(corr ? J.star[1] : J.state[par][1])};
     // The x mean of T, Tm(z) = Re T[kx=0], is O(1) (the conductive profile)
     // while the rest is small: transforming T whole in single precision would
     // spread Tm's rounding over every mode.  Tm is constant along x, so
@@ -504,12 +504,14 @@ __device__ __forceinline__ void fdz_cols_body(const FdzJob<T>& J, int local, int
         const int c0 = 2 * pr;
         const size_t o = (size_t)x * Z + j0 + c0;
         const bool two = c0 + 1 < ncol;
-        if (J.phys[0]) { J.phys[0][o] = vx.x; if (two) J.phys[0][o + 1] = vx.y; }
-        if (J.phys[1]) { J.phys[1][o] = vz.x; if (two) J.phys[1][o + 1] = vz.y; }
-        if (J.phys[2]) { J.phys[2][o] = w.x; if (two) J.phys[2][o + 1] = w.y; }
-        if (J.phys[3]) {
-          J.phys[3][o] = t.x + tmean[c0];
-          if (two) J.phys[3][o + 1] = t.y + tmean[c0 + 1];
+        const gptr<T> p0 = glb(J.phys[0]), p1 = glb(J.phys[1]), p2 = glb(J.phys[2]),
+                      p3 = glb(J.phys[3]);
+        if (p0) { p0[o] = vx.x; if (two) p0[o + 1] = vx.y; }
+        if (p1) { p1[o] = vz.x; if (two) p1[o + 1] = vz.y; }
+        if (p2) { p2[o] = w.x; if (two) p2[o + 1] = w.y; }
+        if (p3) {
+          p3[o] = t.x + tmean[c0];
+          if (two) p3[o + 1] = t.y + tmean[c0 + 1];
         }
       }
       z[0] = mk<T>(vx.x * w.x, vx.y * w.y);              // ux omega
@@ -517,7 +519,7 @@ __device__ __forceinline__ void fdz_cols_body(const FdzJob<T>& J, int local, int
       z[2 * LDSN] = mk<T>(vx.x * t.x, vx.y * t.y);       // ux T'
       z[3 * LDSN] = mk<T>(vz.x * t.x, vz.y * t.y);       // uz T'
     }
-    if (dg) block_diag<kNT>((double)mux, (double)muz, ke, ens, flux, J.diag + (size_t)kDiagPart * local);
+    if (dg) block_diag<kNT>((double)mux, (double)muz, ke, ens, flux, glb(J.diag) + (size_t)kDiagPart * local);
     __syncthreads();
     fft_lds<T, LOGN, -1, kNT, P>(lds, nslot, tw);
     const T hs = (T)(0.5 / (double)N);
@@ -544,10 +546,10 @@ __device__ __forceinline__ void fdz_cols_body(const FdzJob<T>& J, int local, int
         vz.y = T(0);
       }
       const T tm = tmean[cc];
-      J.qw[0][idx] = ov[0];
-      J.qw[1][idx] = ov[1];
-      J.qw[2][idx] = mk<T>(ov[2].x + tm * vx.x, ov[2].y + tm * vx.y);
-      J.qw[3][idx] = mk<T>(ov[3].x + tm * vz.x, ov[3].y + tm * vz.y);
+      glb(J.qw[0])[idx] = ov[0];
+      glb(J.qw[1])[idx] = ov[1];
+      glb(J.qw[2])[idx] = mk<T>(ov[2].x + tm * vx.x, ov[2].y + tm * vx.y);
+      glb(J.qw[3])[idx] = mk<T>(ov[3].x + tm * vz.x, ov[3].y + tm * vz.y);
     }
   }
 }

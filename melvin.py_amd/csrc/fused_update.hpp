@@ -34,8 +34,8 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   const int field = S.eq_field[eqi];
   const int c0 = tile * TC;
   const int ncol = min(TC, nm - c0);
-  const cx<T>* __restrict__ pxc = J.work + (size_t)pchan(J.pbase, eqi, 0) * chs;
-  const cx<T>* __restrict__ pzc = J.work + (size_t)pchan(J.pbase, eqi, 1) * chs;
+  const cx<T>* __restrict__ pxc = glb(J.work) + (size_t)pchan(J.pbase, eqi, 0) * chs;
+  const cx<T>* __restrict__ pzc = glb(J.work) + (size_t)pchan(J.pbase, eqi, 1) * chs;
   // One transform per column: the periodic x difference acts on the
   // intermediate rows (it commutes with the z transform) and the z
   // difference is its modified wavenumber, so
@@ -153,9 +153,9 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   const double hab = q == 2 ? dt / 2 : dt / 24;
   const double coeff = J.coeff[field];
   const T scale = (T)J.scale;
-  const cx<T>* __restrict__ sin_ = J.state[par][field];
-  cx<T>* __restrict__ sout = J.state[par ^ 1][field];
-  cx<T>* __restrict__ hist = J.hist[field];
+  const cx<T>* __restrict__ sin_ = glb(J.state[par][field]);
+  cx<T>* __restrict__ sout = glb(J.state[par ^ 1][field]);
+  cx<T>* __restrict__ hist = glb(J.hist[field]);
   // Epilogue: one wavefront per column, lanes on consecutive rows (the
   // spectral state is kz-major, so a column is contiguous).  Loads of U2
   // row groups are issued before any is used.
@@ -163,7 +163,7 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
   constexpr int U2 = 4;
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const bool ddc = J.problem == MLV_PROB_DDC;
-  uint32_t* __restrict__ carry = sizeof(T) == 4 ? J.comp[field] : nullptr;
+  uint32_t* __restrict__ carry = sizeof(T) == 4 ? glb(J.comp[field]) : nullptr;
   const int prev1 = (cur + q - 1) % q, prev2 = (cur + q - 2) % q, prev3 = (cur + q - 3) % q;
   const float fcoeff = (float)coeff, ffx2 = (float)J.fx2, ffz2 = (float)J.fz2;
   const float fdt = (float)dt, fc2 = (float)c2;
@@ -197,10 +197,10 @@ __device__ __forceinline__ void stepC_body(const JobDev<T>& J, const StageDev& S
           }
           if (ddc) {
             if (field == 0) {
-              e1[u] = J.state[par][1][idx];
-              e2[u] = J.state[par][2][idx];
+              e1[u] = glb(J.state[par][1])[idx];
+              e2[u] = glb(J.state[par][2])[idx];
             } else {
-              e1[u] = J.state[par][0][idx];
+              e1[u] = glb(J.state[par][0])[idx];
             }
           }
         }

@@ -27,9 +27,9 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
   const XinvOp op = S.a[opi];
   const int c0 = tile * TC;
   const int ncol = min(TC, nm - c0);
-  const cx<T>* __restrict__ src = J.state[op.from_new ? (par ^ 1) : par][op.src];
-  cx<T>* __restrict__ dst = J.work + (size_t)op.ch * work_chs(N, nm);
-  cx<T>* __restrict__ psi_out = (op.op == XOP_UX && op.src == 0 && !op.from_new) ? J.psi : nullptr;
+  const cx<T>* __restrict__ src = glb(J.state[op.from_new ? (par ^ 1) : par][op.src]);
+  cx<T>* __restrict__ dst = glb(J.work) + (size_t)op.ch * work_chs(N, nm);
+  cx<T>* __restrict__ psi_out = (op.op == XOP_UX && op.src == 0 && !op.from_new) ? glb(J.psi) : nullptr;
   const double fx = J.fx, fz = J.fz, fx2 = J.fx2, fz2 = J.fz2;
   const float ffx = (float)fx, ffz = (float)fz, ffx2 = (float)fx2, ffz2 = (float)fz2;
   // Each thread owns ITER = P elements; loads are issued U at a time so
@@ -171,8 +171,8 @@ __device__ __forceinline__ void stepA_mixed(const JobDev<T>& J, const StageDev& 
   const bool live = sl < nslot;
   const XinvOp op = S.a[live ? sl % nA : 0];
   const int m = c0 + sl / nA;
-  const cx<T>* __restrict__ src = J.state[op.from_new ? (par ^ 1) : par][op.src];
-  cx<T>* __restrict__ psi_out = (op.op == XOP_UX && op.src == 0 && !op.from_new) ? J.psi : nullptr;
+  const cx<T>* __restrict__ src = glb(J.state[op.from_new ? (par ^ 1) : par][op.src]);
+  cx<T>* __restrict__ psi_out = (op.op == XOP_UX && op.src == 0 && !op.from_new) ? glb(J.psi) : nullptr;
   const float ffx = (float)J.fx, ffz = (float)J.fz, ffx2 = (float)J.fx2, ffz2 = (float)J.fz2;
 #pragma unroll 1
   for (int i0 = 0; i0 < ITER; i0 += U) {
@@ -218,7 +218,7 @@ __device__ __forceinline__ void stepA_mixed(const JobDev<T>& J, const StageDev& 
   // per op: the row pairs' kMixCols columns, x % kXil fastest, then the
   // column, so consecutive lanes store one contiguous run
   for (int o = 0; o < nA; ++o) {
-    cx<T>* __restrict__ dst = J.work + (size_t)S.a[o].ch * work_chs(N, nm);
+    cx<T>* __restrict__ dst = glb(J.work) + (size_t)S.a[o].ch * work_chs(N, nm);
 #pragma unroll 4
     for (int e = threadIdx.x; e < kMixCols * N; e += NT) {
       const int x2 = e % kXil, cc = (e / kXil) % kMixCols, x = (e / (kXil * kMixCols)) * kXil + x2;
@@ -296,7 +296,7 @@ __device__ __forceinline__ void stepM_body(const JobDev<T>& J, const StageDev& S
   // the kx = 0 coefficient of channel op at kz = m (the values kernel A
   // would put in row 0)
   auto coef = [&](const XinvOp& op, int m) -> cx<T> {
-    const cx<T> f = J.state[op.from_new ? (par ^ 1) : par][op.src][(size_t)m * rows];
+    const cx<T> f = glb(J.state[op.from_new ? (par ^ 1) : par][op.src])[(size_t)m * rows];
     if (op.op == XOP_ID) return f;
     if (op.op == XOP_UZ) return czero<T>();  // i kx psi with kx = 0
     float lap = ffz2 * (float)(m * m);
@@ -320,8 +320,8 @@ __device__ __forceinline__ void stepM_body(const JobDev<T>& J, const StageDev& S
   for (int k = nm + threadIdx.x; k <= N - nm; k += kNT) z[fft_pad(k)] = czero<T>();
   __syncthreads();
   fft_lds<T, LOGN, +1, kNT, P>(lds, 1, tw);
-  T* __restrict__ ma = J.mean + (size_t)oa.ch * N;
-  T* __restrict__ mb = J.mean + (size_t)ob.ch * N;
+  T* __restrict__ ma = glb(J.mean) + (size_t)oa.ch * N;
+  T* __restrict__ mb = glb(J.mean) + (size_t)ob.ch * N;
   for (int n = threadIdx.x; n < N; n += kNT) {
     const cx<T> zv = z[fft_pad(n)];
     ma[n] = zv.x;

@@ -141,7 +141,7 @@ __device__ __forceinline__ void bpoint(const JobDev<T>& J, const StageDev& S, bo
   T mv[NIN];
 #pragma unroll
   for (int q = 0; q < NIN; ++q)
-    mv[q] = (kSplit && !Pg::zero_mean(q)) ? J.mean[(size_t)q * N + n] : T(0);
+    mv[q] = (kSplit && !Pg::zero_mean(q)) ? glb(J.mean)[(size_t)q * N + n] : T(0);
   // the whole value of sequence q (row q / NIN, channel q % NIN)
   auto whole = [&](int q) { return kSplit ? mv[q % NIN] + v[q] : v[q]; };
   if (diag) {
@@ -164,7 +164,7 @@ __device__ __forceinline__ void bpoint(const JobDev<T>& J, const StageDev& S, bo
       if (q < nseq) {
 #pragma unroll
         for (int i = 0; i < kMaxPhys; ++i)
-          if (S.phys_ch[i] == q % NIN) J.phys[i][(size_t)(x0 + q / NIN) * N + n] = whole(q);
+          if (S.phys_ch[i] == q % NIN) glb(J.phys[i])[(size_t)(x0 + q / NIN) * N + n] = whole(q);
       }
     }
   }
@@ -230,7 +230,7 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     // slot s is loaded by wavefront s and transformed by the same
     // wavefront: no workgroup barrier before the inverse transform
     constexpr bool WL_IN = fft_tps<N, P>() == 64 && Sh::WPS == 1 && Sh::SL == Sh::NW;
-    bload<T, Sh>(J.work, chs, nm, x0, nseq, nslot, lds);
+    bload<T, Sh>(glb(J.work), chs, nm, x0, nseq, nslot, lds);
     if constexpr (WL_IN)
       wave_sync();
     else
@@ -260,10 +260,10 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
         if (o < nrows * NEQ) lds[o * LDSN + fft_pad(n)] = mk<T>(px[o], pz[o]);
     }
     if (diag) block_diag<kNT>((double)d.mux, (double)d.muz, d.ke, d.ens, d.flux,
-                              J.diag + (size_t)kDiagPart * local);
+                              glb(J.diag) + (size_t)kDiagPart * local);
     __syncthreads();
     fft_lds<T, LOGN, -1, kNT, P, true>(lds, nrows * NEQ, tw);
-    bstore<T, Sh>(J.work, chs, nm, nx, x0, nrows, lds);
+    bstore<T, Sh>(glb(J.work), chs, nm, nx, x0, nrows, lds);
   }
 }
 
@@ -294,7 +294,7 @@ __device__ __forceinline__ void stepB_split(const JobDev<T>& J, const StageDev& 
   const int nseq = nrows * NIN;
   const int nslot = Sh::nslots(nrows);
   constexpr bool WL_IN = fft_tps<N, P>() == 64 && Sh::WPS == 1 && Sh::SL == Sh::NW;
-  bload<T, Sh>(J.work, chs, nm, x0, nseq, nslot, lds);
+  bload<T, Sh>(glb(J.work), chs, nm, x0, nseq, nslot, lds);
   if constexpr (WL_IN)
     wave_sync();
   else
@@ -351,7 +351,7 @@ __device__ __forceinline__ void stepB_split(const JobDev<T>& J, const StageDev& 
     for (int o = 0; o < Sh::NO; ++o) pr[o][r] = mk<T>(px[o], pz[o]);
   }
   if (diag) block_diag<kNT>((double)d.mux, (double)d.muz, d.ke, d.ens, d.flux,
-                            J.diag + (size_t)kDiagPart * local);
+                            glb(J.diag) + (size_t)kDiagPart * local);
   // every thread has read its inputs before any product slot is written
   // over them
   __syncthreads();
@@ -370,7 +370,7 @@ __device__ __forceinline__ void stepB_split(const JobDev<T>& J, const StageDev& 
   fft_pass<T, LOGN, 4, LOGR, -1, kNT, P, true>(lds, nrows * NEQ, tw);
   fft_pass<T, LOGN, 4, LOGR + 4, -1, kNT, P, true>(lds, nrows * NEQ, tw);
   __syncthreads();
-  bstore<T, Sh>(J.work, chs, nm, nx, x0, nrows, lds);
+  bstore<T, Sh>(glb(J.work), chs, nm, nx, x0, nrows, lds);
 }
 
 template <typename T, int LOGN, int V>
