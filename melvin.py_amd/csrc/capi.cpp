@@ -1025,7 +1025,8 @@ static int plan_batch(int dtype, const mlv_job_desc* jobs, int njobs, int device
       return (kNT * p) >> lognz;
     };
     auto zlds = [&](int v) {
-      return (size_t)zslots(v) * fft_ldsn(1 << lognz) * (f64 ? 16 : 8);
+      return (size_t)bslots_used(bvar_nin(v), bvar_neq(v), zslots(v)) * fft_ldsn(1 << lognz) *
+             (f64 ? 16 : 8);
     };
     size_t ldsz = 0;
     for (int s = 0; s < nstage; ++s) ldsz = std::max(ldsz, zlds(bvar[s]));

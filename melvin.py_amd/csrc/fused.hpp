@@ -311,6 +311,19 @@ __host__ __device__ constexpr size_t xrow_base(int x, int nm) {
   return (size_t)(x / kXil) * (kXil * (size_t)nm) + (size_t)(x % kXil);
 }
 
+// LDS slots a z-pass block of `slots` available slots touches: its input
+// slots (a row pair's channel per slot when the row count is even, as
+// BShape lays them out) or its product slots, whichever is more.  The
+// launch sizes the LDS to these alone, so a slot the block never uses does
+// not cost blocks per CU (3 of 4 slots for a one-field vortex at 1024 z
+// points: 5 blocks per CU instead of 4).
+__host__ __device__ constexpr int bslots_used(int nin, int neq, int slots) {
+  const int rb = brows(nin, neq, slots);
+  const int in = (rb % 2 == 0 && kXil == 2) ? nin * (rb / 2) : (rb * nin + 1) / 2;
+  const int out = rb * neq;
+  return in > out ? in : out;
+}
+
 // Layout of the Px / Pz channels (kernel B writes them, kernel C reads
 // them): kz in groups of kPGroup, each group x-major, so element (x, k) is
 // at pidx(x, k, nx).  A column tile of kernel C (kPGroup columns, or half

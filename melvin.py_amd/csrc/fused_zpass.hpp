@@ -44,6 +44,8 @@ struct BShape {
   }
   static constexpr int NO = RB * NEQ;                  // product slots
   static constexpr int WPO = NO >= NW ? 1 : NW / NO;   // wavefronts per product slot
+  static_assert(SLN <= bslots_used(NIN, NEQ, SL) && NO <= bslots_used(NIN, NEQ, SL),
+                "the launch's LDS holds the slots in use");
 };
 
 // Two consecutive elements (16 bytes in single precision, one load).
