@@ -17,11 +17,11 @@ using namespace mlv::abi;
 
 namespace {
 
-// Row kernel variant of a job with K elements per lane: the smallest
-// compiled size >= K.
-int fdz_variant(int K) {
+// Row kernel variant of a job with wpr wavefronts per row and K elements
+// per thread: the smallest compiled size >= K of that wpr.
+int fdz_variant(int K, int wpr) {
   for (int v = 0; v < kFdzVariants; ++v)
-    if (kFdzKts[v] >= K) return v;
+    if (kFdzWprs[v] == wpr && kFdzKts[v] >= K) return v;
   return -1;
 }
 
@@ -93,7 +93,7 @@ int build_fdz_job(const mlv_fdz_job_desc& d, FdzJob<T>& J, int j) {
   std::memset(&J, 0, sizeof(J));
   const int lnx = ilog2_exact(d.nx);
   if (lnx < 3 || lnx > 11) return bad("nx must be a power of two in [8, 2048]");
-  if (d.nz < 5 || d.nz > 64 * kFdzMaxK) return bad("nz must be in [5, 768]");
+  if (d.nz < 5 || d.nz > kFdzMaxNz) return bad("nz must be in [5, 768]");
   if (d.fd_order != 2 && d.fd_order != 4) return bad("fd_order must be 2 or 4");
   if (d.ab_order != 2 && d.ab_order != 4) return bad("ab_order must be 2 or 4");
   if (d.scheme != MLV_FDZ_EXPLICIT && d.scheme != MLV_FDZ_SEMI_IMPLICIT && d.scheme != MLV_FDZ_PC)
@@ -107,8 +107,9 @@ int build_fdz_job(const mlv_fdz_job_desc& d, FdzJob<T>& J, int j) {
   J.nz = d.nz;
   J.lognx = lnx;
   J.nn = (d.nx - 1) / 3;
-  J.K = (d.nz + 63) / 64;
-  J.kt = kFdzKts[fdz_variant(J.K)];
+  J.wpr = fdz_wpr(d.nz);
+  J.K = (d.nz + 64 * J.wpr - 1) / (64 * J.wpr);
+  J.kt = kFdzKts[fdz_variant(J.K, J.wpr)];
   J.fd_order = d.fd_order;
   J.scheme = d.scheme;
   J.q = d.ab_order;
@@ -157,11 +158,11 @@ hipError_t job_factors(FdzHost* b, int j, bool all, hipStream_t s) {
   const FdzJob<T>& J = host_job<T>(b, j);
   const double fxabs = J.fx;
   hipError_t e = hipSuccess;
-  if (all) e = fdz_factors(J.fac[0], J.nn, J.nz, J.K, fxabs, J.dz, 0, 0.0, s);
+  if (all) e = fdz_factors(J.fac[0], J.nn, J.nz, J.K, J.wpr, fxabs, J.dz, 0, 0.0, s);
   if (e == hipSuccess && J.scheme != MLV_FDZ_EXPLICIT) {
-    e = fdz_factors(J.fac[1], J.nn, J.nz, J.K, fxabs, J.dz, 1, J.alpha * J.dt * J.Pr, s);
+    e = fdz_factors(J.fac[1], J.nn, J.nz, J.K, J.wpr, fxabs, J.dz, 1, J.alpha * J.dt * J.Pr, s);
     if (e == hipSuccess)
-      e = fdz_factors(J.fac[2], J.nn, J.nz, J.K, fxabs, J.dz, 2, J.alpha * J.dt * 1.0, s);
+      e = fdz_factors(J.fac[2], J.nn, J.nz, J.K, J.wpr, fxabs, J.dz, 2, J.alpha * J.dt * 1.0, s);
   }
   return e;
 }
@@ -191,10 +192,11 @@ int create_impl(FdzHost* b, const mlv_fdz_job_desc* jobs, int njobs) {
     zmax = std::max(zmax, J.nz);
     const int nph = J.scheme == MLV_FDZ_PC ? 2 : 1;
     b->nphase = std::max(b->nphase, nph);
-    const int var = fdz_variant(J.K);
+    const int var = fdz_variant(J.K, J.wpr);
     const int cb = (J.nz + tc - 1) / tc;
-    const int rb2 = (2 * J.nn + kFdzWaves - 1) / kFdzWaves;
-    const int rb1 = (J.nn + kFdzWaves - 1) / kFdzWaves;
+    const int rpb = fdz_rows_per_block(J.wpr);
+    const int rb2 = (2 * J.nn + rpb - 1) / rpb;
+    const int rb1 = (J.nn + rpb - 1) / rpb;
     for (int p = 0; p < 2; ++p) pcol[p][j + 1] = pcol[p][j] + (p < nph ? cb : 0);
     J.nbC = cb;
     b->max_nn = std::max(b->max_nn, J.nn);
@@ -204,8 +206,12 @@ int create_impl(FdzHost* b, const mlv_fdz_job_desc* jobs, int njobs) {
     }
   }
   fl.zstride = zmax | 1;  // odd stride: the waves' rows start on different banks
-  fl.ldsrow = (size_t)kFdzWaves * 2 * fl.zstride * sizeof(double2);
-  if (fl.ldsrow > lds_max) return fail(MLV_ERR_UNSUPPORTED, "nz too large for the row kernel LDS");
+  // two LDS rows (buf, vrow) per row of a block
+  for (int v = 0; v < kFdzVariants; ++v) {
+    fl.ldsrow[v] = (size_t)fdz_rows_per_block(kFdzWprs[v]) * 2 * fl.zstride * sizeof(double2);
+    if (fl.ldsrow[v] > lds_max)
+      return fail(MLV_ERR_UNSUPPORTED, "nz too large for the row kernel LDS");
+  }
   auto dalloc = [&](size_t bytes, void** p) -> int {
     hipError_t e = hipMalloc(p, bytes);
     if (e != hipSuccess) return hip_fail(e, "hipMalloc fdz batch");
@@ -235,7 +241,7 @@ int create_impl(FdzHost* b, const mlv_fdz_job_desc* jobs, int njobs) {
   // factor tables, chunk-major, one (cp, 1/den) pair per element
   for (int j = 0; j < njobs; ++j) {
     FdzJob<T>& J = host_job<T>(b, j);
-    const size_t per = (size_t)J.nn * J.K * 64 * sizeof(double2);
+    const size_t per = (size_t)J.nn * J.K * 64 * J.wpr * sizeof(double2);
     const int nops = J.scheme == MLV_FDZ_EXPLICIT ? 1 : 3;
     for (int o = 0; o < nops; ++o) {
       int r = dalloc(per, (void**)&J.fac[o]);

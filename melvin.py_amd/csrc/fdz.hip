@@ -17,31 +17,50 @@ __device__ __forceinline__ cx<T> t2(double2 v) {
 }
 
 // --------------------------------------------------------------------------
-// Tridiagonal solve of one row by one wavefront, FP64.
+// Tridiagonal solve of one row by its WPR wavefronts, FP64.
 //
 // The system has identity first and last rows and constant interior bands
 // (lower = upper = lo).  fac holds, for element j, (cp_j, 1/den_j) of the
-// Thomas factorisation, stored chunk-major: entry i * 64 + lane is element
-// j = lane * K + i.  Lane l owns elements [l K, l K + K).
+// Thomas factorisation, stored chunk-major: entry i * LR + t is element
+// j = t K + i, LR = 64 WPR threads per row.  Thread t owns elements
+// [t K, t K + K).
 //   forward   d_j = (r_j - lower_j d_{j-1}) / den_j
 //   backward  x_j = d_j - cp_j x_{j+1},  x_{nz-1} = d_{nz-1}
-// Each recurrence is an affine map per element; a lane composes the maps of
-// its chunk, an inclusive shuffle scan composes them across lanes, and the
-// lane then replays its chunk from the value entering it.  buf holds the
-// right-hand side on entry and the solution on exit.
+// Each recurrence is an affine map per element; a thread composes the maps
+// of its chunk, an inclusive shuffle scan composes them across a
+// wavefront's lanes, and the thread then replays its chunk from the value
+// entering it.  With two wavefronts per row the value leaving the first
+// wavefront's last element (forward) or entering the second's first
+// element (backward) goes through xch (4 doubles of LDS) between them.
+// buf holds the right-hand side on entry and the solution on exit.
 // --------------------------------------------------------------------------
-template <int KT>
-__device__ __forceinline__ void wave_tridiag(double2* __restrict__ buf,
-                                             const double2* __restrict__ fac, double lo, int nz,
-                                             int K, int lane) {
-  const int j0 = lane * K;
+template <int WPR>
+__device__ __forceinline__ void row_sync() {
+  if constexpr (WPR == 1)
+    wave_sync();
+  else
+    __syncthreads();  // a block holds exactly one row
+}
+
+template <int KT, int WPR>
+__device__ __forceinline__ void row_tridiag(double2* __restrict__ buf,
+                                            gptr<const double2> fac, double lo, int nz, int K,
+                                            int t, double* __restrict__ xch) {
+  static_assert(WPR == 1 || WPR == 2, "one or two wavefronts per row");
+  constexpr int LR = 64 * WPR;
+  const int lane = t & 63, wv = t >> 6;
+  const int j0 = t * K;
   double2 d[KT], fc[KT];
 #pragma unroll
   for (int i = 0; i < KT; ++i) {
     const int j = j0 + i;
     const bool ok = i < K && j < nz;
-    fc[i] = ok ? fac[i * 64 + lane] : make_double2(0.0, 1.0);
-    d[i] = ok ? buf[j] : make_double2(0.0, 0.0);
+    fc[i] = make_double2(0.0, 1.0);
+    d[i] = make_double2(0.0, 0.0);
+    if (ok) {
+      fc[i] = fac[i * LR + t];
+      d[i] = buf[j];
+    }
   }
   // forward elimination
   double A = 1.0, Bx = 0.0, By = 0.0;
@@ -69,6 +88,20 @@ __device__ __forceinline__ void wave_tridiag(double2* __restrict__ buf,
   }
   double px = __shfl_up(Bx, 1, 64), py = __shfl_up(By, 1, 64);
   if (lane == 0) px = py = 0.0;
+  if constexpr (WPR == 2) {
+    // the second wavefront's lanes continue from the first's last value
+    double ax = __shfl_up(A, 1, 64);
+    if (lane == 0) ax = 1.0;
+    if (t == 63) {
+      xch[0] = Bx;
+      xch[1] = By;
+    }
+    __syncthreads();
+    if (wv == 1) {
+      px = ax * xch[0] + px;
+      py = ax * xch[1] + py;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < KT; ++i) {
     const int j = j0 + i;
@@ -107,6 +140,20 @@ __device__ __forceinline__ void wave_tridiag(double2* __restrict__ buf,
   }
   double xx = __shfl_down(Bx, 1, 64), xy = __shfl_down(By, 1, 64);
   if (lane == 63) xx = xy = 0.0;
+  if constexpr (WPR == 2) {
+    // the first wavefront's lanes continue from the second's first value
+    double ax = __shfl_down(A, 1, 64);
+    if (lane == 63) ax = 1.0;
+    if (t == 64) {
+      xch[2] = Bx;
+      xch[3] = By;
+    }
+    __syncthreads();
+    if (wv == 0) {
+      xx = ax * xch[2] + xx;
+      xy = ax * xch[3] + xy;
+    }
+  }
 #pragma unroll
   for (int i = KT - 1; i >= 0; --i) {
     const int j = j0 + i;
@@ -153,15 +200,17 @@ __device__ __forceinline__ double2 d2dz2_row(const double2* __restrict__ b, int 
 }
 
 // --------------------------------------------------------------------------
-// Row kernel body: one wavefront, mode n, field f (0 omega, 1 T).  Element-
-// wise work uses the coalesced layout j = lane + 64 i.  The wave owns two
-// LDS rows of FP64 complex values: buf (operands with neighbours, and the
-// tridiagonal solves) and vrow (the state v_n the update starts from).
+// Row kernel body: mode n, field f (0 omega, 1 T), by the row's WPR
+// wavefronts (thread t of 64 WPR).  Element-wise work uses the coalesced
+// layout j = t + 64 WPR i.  The row owns two LDS rows of FP64 complex
+// values: buf (operands with neighbours, and the tridiagonal solves) and
+// vrow (the state v_n the update starts from).
 // --------------------------------------------------------------------------
-template <typename T, int KT>
+template <typename T, int KT, int WPR>
 __device__ __forceinline__ void fdz_row(const FdzJob<T>& J, int stage, int n, int f, int step,
-                        double2* __restrict__ buf, double2* __restrict__ vrow) {
-  const int lane = threadIdx.x & 63;
+                                        double2* __restrict__ buf, double2* __restrict__ vrow,
+                                        int t, double* __restrict__ xch) {
+  constexpr int LR = 64 * WPR;  // threads per row
   const int Z = J.nz, K = J.K;
   const int par = (J.par0 + step) & 1;
   const size_t row = (size_t)n * Z;
@@ -178,7 +227,7 @@ __device__ __forceinline__ void fdz_row(const FdzJob<T>& J, int stage, int n, in
     const cx<T>* __restrict__ cin = J.comp[f] ? glb(J.comp[f]) + row : nullptr;
 #pragma unroll
     for (int i = 0; i < KT; ++i) {
-      const int j = lane + 64 * i;
+      const int j = t + LR * i;
       if (i < K && j < Z) {
         buf[j] = d2<T>(qz[j]);
         double2 v = d2<T>(vin[j]);
@@ -190,7 +239,7 @@ __device__ __forceinline__ void fdz_row(const FdzJob<T>& J, int stage, int n, in
         vrow[j] = v;
       }
     }
-    wave_sync();
+    row_sync<WPR>();
     double sx;
     {
       const double th = 2.0 * M_PI * (double)n / (double)J.nx;
@@ -211,7 +260,7 @@ __device__ __forceinline__ void fdz_row(const FdzJob<T>& J, int stage, int n, in
     double2 inc[KT];
 #pragma unroll
     for (int i = 0; i < KT; ++i) {
-      const int j = lane + 64 * i;
+      const int j = t + LR * i;
       inc[i] = make_double2(0.0, 0.0);
       if (i < K && j < Z) {
         const size_t e = row + j;
@@ -262,26 +311,27 @@ __device__ __forceinline__ void fdz_row(const FdzJob<T>& J, int stage, int n, in
         inc[i] = expl ? E : make_double2(dt * coeff * lv.x + E.x, dt * coeff * lv.y + E.y);
       }
     }
-    wave_sync();
+    row_sync<WPR>();
 #pragma unroll
     for (int i = 0; i < KT; ++i) {
-      const int j = lane + 64 * i;
+      const int j = t + LR * i;
       if (i < K && j < Z) buf[j] = inc[i];
     }
     if (!expl) {
       // increment form of the theta scheme:
       //   (I - a dt c L) delta = dt c L v + E,  v_new = v + delta
-      wave_sync();
-      wave_tridiag<KT>(buf, glb(J.fac[1 + f]) + (size_t)n * K * 64, J.lo[1 + f], Z, K, lane);
+      row_sync<WPR>();
+      row_tridiag<KT, WPR>(buf, glb(J.fac[1 + f]) + (size_t)n * K * LR, J.lo[1 + f], Z, K,
+                           t, xch);
     }
-    wave_sync();
+    row_sync<WPR>();
     // new state and wall writes (rayleigh_benard_convection.py:113-142);
-    // buf[j] becomes -omega_new for the Poisson solve (same lane, same j)
+    // buf[j] becomes -omega_new for the Poisson solve (same thread, same j)
     cx<T>* __restrict__ out = glb(stage == FDZ_PRED ? J.star[f] : J.state[par ^ 1][f]) + row;
     cx<T>* __restrict__ cout = (stage != FDZ_PRED && J.comp[f]) ? glb(J.comp[f]) + row : nullptr;
 #pragma unroll
     for (int i = 0; i < KT; ++i) {
-      const int j = lane + 64 * i;
+      const int j = t + LR * i;
       if (i < K && j < Z) {
         const double2 dl = buf[j], v0 = vrow[j];
         double2 v = make_double2(v0.x + dl.x, v0.y + dl.y);
@@ -309,7 +359,7 @@ __device__ __forceinline__ void fdz_row(const FdzJob<T>& J, int stage, int n, in
     const cx<T>* __restrict__ cin = J.comp[0] ? glb(J.comp[0]) + row : nullptr;
 #pragma unroll
     for (int i = 0; i < KT; ++i) {
-      const int j = lane + 64 * i;
+      const int j = t + LR * i;
       if (i < K && j < Z) {
         double2 w = d2<T>(vin[j]);
         if (cin) {
@@ -324,9 +374,9 @@ __device__ __forceinline__ void fdz_row(const FdzJob<T>& J, int stage, int n, in
   if (f != 0) return;
   // psi = solve(-omega) (utility.py:65, LaplacianSolver.py:70-78),
   // ux_s = -pddz(psi), uz_s = i kx psi (utility.py:66-79)
-  wave_sync();
-  wave_tridiag<KT>(buf, glb(J.fac[0]) + (size_t)n * K * 64, J.lo[0], Z, K, lane);
-  wave_sync();
+  row_sync<WPR>();
+  row_tridiag<KT, WPR>(buf, glb(J.fac[0]) + (size_t)n * K * LR, J.lo[0], Z, K, t, xch);
+  row_sync<WPR>();
   const int vs = stage == FDZ_PRED ? 2 : (stage == FDZ_PROLOGUE ? par : par ^ 1);
   cx<T>* __restrict__ ux = glb(J.vel[vs][0]) + row;
   cx<T>* __restrict__ uz = glb(J.vel[vs][1]) + row;
@@ -334,7 +384,7 @@ __device__ __forceinline__ void fdz_row(const FdzJob<T>& J, int stage, int n, in
   const double kx = J.fx * (double)n;
 #pragma unroll
   for (int i = 0; i < KT; ++i) {
-    const int j = lane + 64 * i;
+    const int j = t + LR * i;
     if (i < K && j < Z) {
       const double2 p = buf[j];
       const double2 dd = ddz_row(buf, j, Z, order, ih);
@@ -345,17 +395,20 @@ __device__ __forceinline__ void fdz_row(const FdzJob<T>& J, int stage, int n, in
   }
 }
 
-template <typename T, int KT>
-__global__ void __launch_bounds__(64 * kFdzWaves)
+// Blocks of fdz_rows_per_block(WPR) rows of WPR wavefronts each.
+template <typename T, int KT, int WPR>
+__global__ void __launch_bounds__(fdz_row_threads(WPR))
     k_fdz_rows(const FdzJob<T>* __restrict__ jobs, FdzJob<T>* clock, const int* __restrict__ pref,
                int njobs, int phase, int step, int zstride) {
   extern __shared__ char smem_raw[];
+  __shared__ double xch[4];
   const int lb = xcd_logical(pref[njobs]);
   if (lb < 0) return;
   const int jb = find_job(pref, njobs, lb);
   const FdzJob<T>& J = jobs[jb];
-  const int wave = threadIdx.x >> 6;
-  const int lw = (lb - pref[jb]) * kFdzWaves + wave;
+  constexpr int LR = 64 * WPR;
+  const int rib = threadIdx.x / LR, t = threadIdx.x % LR;  // row in block, thread in row
+  const int lw = (lb - pref[jb]) * fdz_rows_per_block(WPR) + rib;
   int stage, nwf;
   if (phase < 0) {
     stage = FDZ_PROLOGUE;
@@ -371,10 +424,11 @@ __global__ void __launch_bounds__(64 * kFdzWaves)
     clock[jb].t = clock[jb].t + J.dt;
     clock[jb].loops = clock[jb].loops + 1;
   }
-  if (lw >= nwf * J.nn || J.kt != KT) return;
+  // (uniform over a block when WPR > 1: the block is one row)
+  if (lw >= nwf * J.nn || J.kt != KT || J.wpr != WPR) return;
   const int n = lw / nwf, f = lw % nwf;
-  double2* buf = reinterpret_cast<double2*>(smem_raw) + (size_t)(2 * wave) * zstride;
-  fdz_row<T, KT>(J, stage, n, f, step, buf, buf + zstride);
+  double2* buf = reinterpret_cast<double2*>(smem_raw) + (size_t)(2 * rib) * zstride;
+  fdz_row<T, KT, WPR>(J, stage, n, f, step, buf, buf + zstride, t, xch);
 }
 
 // --------------------------------------------------------------------------
@@ -577,23 +631,24 @@ __global__ void __launch_bounds__(kNT)
   }
 }
 
-// Thomas factors of one job's operator for mode n, by one wavefront (FP64).
+// Thomas factors of one job's operator for mode n, by lr = 64 wpr threads
+// (FP64), in the row kernel's layout.
 //   which = 0: Poisson, interior bands [1/dz^2, -(kx^2 + 2/dz^2), 1/dz^2]
 //              (LaplacianSolver.py:21-52)
 //   which > 0: theta diffusion, [-a/dz^2, 1 + a (kx^2 + 2/dz^2), -a/dz^2]
-// First and last rows (and the padding up to 64 K) are identity rows.  The
-// table is chunk-major (entry i * 64 + lane is element lane * K + i): lane
-// l computes its chunk [l K, l K + K) and stores entry i of it with the
-// other lanes' entries i, so each store instruction writes 64 consecutive
+// First and last rows (and the padding up to lr K) are identity rows.  The
+// table is chunk-major (entry i * lr + t is element t K + i): thread t
+// computes its chunk [t K, t K + K) and stores entry i of it with the
+// other threads' entries i, so each store instruction writes consecutive
 // entries.  The interior recurrence cp_j = lo / (dg - lo cp_{j-1}) has
-// constant coefficients: a lane runs it from the first row up to its
+// constant coefficients: a thread runs it from the first row up to its
 // chunk, stopping once it maps a value to itself in floating point (every
 // later factor is then that value; diffusion gets there within a few dozen
 // rows), and then through its chunk -- the same operations in the same
 // order as a single sequential pass, so the table is bit-identical to one.
-__device__ void fdz_factor_wave(double2* __restrict__ fac, int n, int nz, int K, double fxabs,
-                                double dz, int which, double a, int lane) {
-  double2* F = fac + (size_t)n * K * 64;
+__device__ void fdz_factor_row(double2* __restrict__ fac, int n, int nz, int K, int lr,
+                               double fxabs, double dz, int which, double a, int t) {
+  gptr<double2> F = glb(fac) + (size_t)n * K * lr;
   const double k = (double)n * fxabs;
   const double k2 = k * k;
   const double idz2 = 1.0 / (dz * dz);
@@ -605,7 +660,7 @@ __device__ void fdz_factor_wave(double2* __restrict__ fac, int n, int nz, int K,
     lo = -a * idz2;
     dg = 1.0 + a * (k2 + 2.0 / (dz * dz));
   }
-  const int j0 = lane * K;
+  const int j0 = t * K;
   // factors of element j - 1 before element j (row 0: cp = 0)
   double cp = 0.0, rp = 0.0;
   for (int j = 1; j < j0 && j < nz - 1; ++j) {
@@ -626,15 +681,16 @@ __device__ void fdz_factor_wave(double2* __restrict__ fac, int n, int nz, int K,
       rp = rd;
       v = make_double2(c, rd);
     }
-    F[i * 64 + lane] = v;
+    F[i * lr + t] = v;
   }
 }
 
-// Grid: one 64-thread block per mode.
-__global__ void __launch_bounds__(64) k_fdz_factors(double2* __restrict__ fac, int nn, int nz,
-                                                    int K, double fxabs, double dz, int which,
-                                                    double a) {
-  if ((int)blockIdx.x < nn) fdz_factor_wave(fac, blockIdx.x, nz, K, fxabs, dz, which, a, threadIdx.x);
+// Grid: one block of lr threads per mode.
+__global__ void __launch_bounds__(128) k_fdz_factors(double2* __restrict__ fac, int nn, int nz,
+                                                     int K, int lr, double fxabs, double dz,
+                                                     int which, double a) {
+  if ((int)blockIdx.x < nn && (int)threadIdx.x < lr)
+    fdz_factor_row(fac, blockIdx.x, nz, K, lr, fxabs, dz, which, a, threadIdx.x);
 }
 
 // One block per job: tracker record and CFL rule (diag_apply) from the
@@ -663,16 +719,16 @@ __global__ void __launch_bounds__(kNT) k_fdz_diag(FdzJob<T>* jobs, double cutoff
 }
 
 // Theta tables of the jobs whose dt the last CFL update changed: grid
-// (modes, jobs, 2 operators) of 64-thread blocks.
+// (modes, jobs, 2 operators) of 128-thread blocks, 64 wpr of them used.
 template <typename T>
-__global__ void __launch_bounds__(64) k_fdz_refactor(const FdzJob<T>* __restrict__ jobs) {
+__global__ void __launch_bounds__(128) k_fdz_refactor(const FdzJob<T>* __restrict__ jobs) {
   const FdzJob<T>& J = jobs[blockIdx.y];
-  if (!J.refactor || J.scheme == MLV_FDZ_EXPLICIT) return;
+  if (!J.refactor || J.scheme == MLV_FDZ_EXPLICIT || (int)threadIdx.x >= 64 * J.wpr) return;
   const int n = blockIdx.x;
   if (n >= J.nn) return;
   const int which = 1 + (int)blockIdx.z;
   const double a = J.alpha * J.dt * (which == 1 ? J.Pr : 1.0);
-  fdz_factor_wave(J.fac[which], n, J.nz, J.K, J.fx, J.dz, which, a, threadIdx.x);
+  fdz_factor_row(J.fac[which], n, J.nz, J.K, 64 * J.wpr, J.fx, J.dz, which, a, threadIdx.x);
 }
 
 template <typename T>
@@ -691,11 +747,11 @@ template <typename T, int V>
 static void launch_rows(const FdzJob<T>* jobs, const FdzLaunch& fl, const int* pref, int nblocks,
                         int phase, int step, hipStream_t s) {
   if (nblocks <= 0) return;
-  constexpr int KT = kFdzKts[V];
-  MLV_SET_LDS_ONCE((k_fdz_rows<T, KT>), fl.ldsrow);
-  hipLaunchKernelGGL((k_fdz_rows<T, KT>), dim3(xcd_grid(nblocks)), dim3(64 * kFdzWaves), fl.ldsrow,
-                     s, jobs, const_cast<FdzJob<T>*>(jobs), pref, fl.njobs, phase, step,
-                     fl.zstride);
+  constexpr int KT = kFdzKts[V], WPR = kFdzWprs[V];
+  MLV_SET_LDS_ONCE((k_fdz_rows<T, KT, WPR>), fl.ldsrow[V]);
+  hipLaunchKernelGGL((k_fdz_rows<T, KT, WPR>), dim3(xcd_grid(nblocks)), dim3(fdz_row_threads(WPR)),
+                     fl.ldsrow[V], s, jobs, const_cast<FdzJob<T>*>(jobs), pref, fl.njobs, phase,
+                     step, fl.zstride);
 }
 
 template <typename T>
@@ -707,8 +763,7 @@ static void launch_rows_all(const FdzJob<T>* jobs, const FdzLaunch& fl, int phas
   launch_rows<T, 2>(jobs, fl, fl.prefrow[p][2], fl.nrow[p][2], phase, step, s);
   launch_rows<T, 3>(jobs, fl, fl.prefrow[p][3], fl.nrow[p][3], phase, step, s);
   launch_rows<T, 4>(jobs, fl, fl.prefrow[p][4], fl.nrow[p][4], phase, step, s);
-  launch_rows<T, 5>(jobs, fl, fl.prefrow[p][5], fl.nrow[p][5], phase, step, s);
-  static_assert(kFdzVariants == 6, "row kernel variants");
+  static_assert(kFdzVariants == 5, "row kernel variants");
 }
 
 template <typename T>
@@ -732,10 +787,10 @@ hipError_t fdz_prologue(const FdzJob<T>* jobs, const FdzLaunch& fl, int step, hi
   return hipGetLastError();
 }
 
-hipError_t fdz_factors(double2* fac, int nn, int nz, int K, double fxabs, double dz, int which,
-                       double a, hipStream_t s) {
-  hipLaunchKernelGGL(k_fdz_factors, dim3(nn), dim3(64), 0, s, fac, nn, nz, K, fxabs, dz, which,
-                     a);
+hipError_t fdz_factors(double2* fac, int nn, int nz, int K, int wpr, double fxabs, double dz,
+                       int which, double a, hipStream_t s) {
+  hipLaunchKernelGGL(k_fdz_factors, dim3(nn), dim3(64 * wpr), 0, s, fac, nn, nz, K, 64 * wpr,
+                     fxabs, dz, which, a);
   return hipGetLastError();
 }
 
@@ -750,7 +805,7 @@ template <typename T>
 hipError_t fdz_diag(FdzJob<T>* jobs, int njobs, int max_nn, double cutoff, int cadence,
                     hipStream_t s) {
   hipLaunchKernelGGL(k_fdz_diag<T>, dim3(njobs), dim3(kNT), 0, s, jobs, cutoff, cadence);
-  hipLaunchKernelGGL(k_fdz_refactor<T>, dim3(max_nn, njobs, 2), dim3(64), 0, s,
+  hipLaunchKernelGGL(k_fdz_refactor<T>, dim3(max_nn, njobs, 2), dim3(128), 0, s,
                      (const FdzJob<T>*)jobs);
   return hipGetLastError();
 }
@@ -758,14 +813,10 @@ hipError_t fdz_diag(FdzJob<T>* jobs, int njobs, int max_nn, double cutoff, int c
 template <typename T>
 hipError_t fdz_prepare(const FdzLaunch& fl) {
   if (fl.ldscol) MLV_SET_LDS_ONCE(k_fdz_cols<T>, fl.ldscol);
-  if (fl.ldsrow) {
-    MLV_SET_LDS_ONCE((k_fdz_rows<T, kFdzKts[0]>), fl.ldsrow);
-    MLV_SET_LDS_ONCE((k_fdz_rows<T, kFdzKts[1]>), fl.ldsrow);
-    MLV_SET_LDS_ONCE((k_fdz_rows<T, kFdzKts[2]>), fl.ldsrow);
-    MLV_SET_LDS_ONCE((k_fdz_rows<T, kFdzKts[3]>), fl.ldsrow);
-    MLV_SET_LDS_ONCE((k_fdz_rows<T, kFdzKts[4]>), fl.ldsrow);
-    MLV_SET_LDS_ONCE((k_fdz_rows<T, kFdzKts[5]>), fl.ldsrow);
-  }
+#define MLV_ROWS_LDS(V) \
+  if (fl.ldsrow[V]) MLV_SET_LDS_ONCE((k_fdz_rows<T, kFdzKts[V], kFdzWprs[V]>), fl.ldsrow[V]);
+  MLV_ROWS_LDS(0) MLV_ROWS_LDS(1) MLV_ROWS_LDS(2) MLV_ROWS_LDS(3) MLV_ROWS_LDS(4)
+#undef MLV_ROWS_LDS
   return hipGetLastError();
 }
 

@@ -36,17 +36,29 @@
 namespace mlv {
 
 enum FdzStage : int { FDZ_MAIN = 0, FDZ_PRED = 1, FDZ_CORR = 2, FDZ_PROLOGUE = 3 };
-constexpr int kFdzWaves = 4;   // wavefronts (mode, field) per row block
-// Row elements per lane (K = ceil(nz / 64)) the row kernel is compiled for;
-// a job runs in the smallest variant >= its K.  nz <= 64 * 12: beyond that
-// the per-lane arrays no longer fit the register file.
-constexpr int kFdzVariants = 6;
-constexpr int kFdzKts[kFdzVariants] = {1, 2, 4, 6, 9, 12};
-constexpr int kFdzMaxK = 12;
+constexpr int kFdzWaves = 4;   // rows (mode, field) per row block of one wavefront per row
+// Row kernel variants: KT row elements per thread and WPR wavefronts per
+// row.  A row of nz <= 256 points takes one wavefront (K = ceil(nz / 64)
+// elements per lane, blocks of kFdzWaves rows); a longer row takes two
+// (K = ceil(nz / 128), a block per row), which halves each wavefront's
+// serial chain of instructions -- a single job has fewer rows than the
+// device has SIMDs, so that chain sets the kernel's time.  The choice
+// depends on the job alone, so a job computes the same bits in any batch.
+// A job runs in the smallest variant of its WPR with KT >= its K.
+constexpr int kFdzVariants = 5;
+constexpr int kFdzKts[kFdzVariants] = {1, 2, 4, 4, 6};
+constexpr int kFdzWprs[kFdzVariants] = {1, 1, 1, 2, 2};
+constexpr int kFdzMaxNz = 768;
+__host__ __device__ constexpr int fdz_wpr(int nz) { return nz > 256 ? 2 : 1; }
+// rows per block and threads per block of a variant with wpr wavefronts per row
+__host__ __device__ constexpr int fdz_rows_per_block(int wpr) { return wpr == 1 ? kFdzWaves : 1; }
+__host__ __device__ constexpr int fdz_row_threads(int wpr) {
+  return 64 * wpr * fdz_rows_per_block(wpr);
+}
 
 template <typename T>
 struct FdzJob {
-  int nx, nz, lognx, nn, K, kt, fd_order, scheme, q, curr0, par0;
+  int nx, nz, lognx, nn, K, kt, wpr, fd_order, scheme, q, curr0, par0;
   double fx, fx2, dx, dz, dt, alpha, Pr, Ra;
   double lo[3];            // interior off-diagonal of each tridiagonal operator
   cx<T>* state[2][2];      // [buffer][omega, T]
@@ -78,7 +90,7 @@ struct FdzLaunch {
   int njobs;
   int ncol[2];                       // column blocks of phases 0 and 1
   int nrow[3][kFdzVariants];         // row blocks of phases 0, 1 and the prologue
-  size_t ldsrow, ldscol;
+  size_t ldsrow[kFdzVariants], ldscol;  // LDS of each row variant's blocks
   int zstride;                       // LDS elements per row of the row kernel
   int *prefcol[2];
   int *prefrow[3][kFdzVariants];
@@ -117,8 +129,8 @@ template <typename T>
 hipError_t fdz_prologue(const FdzJob<T>* jobs, const FdzLaunch& fl, int step, hipStream_t s);
 // (Re)build the factor tables of one job: which = 0 Poisson, 1 theta(omega),
 // 2 theta(T); theta tables depend on dt.
-hipError_t fdz_factors(double2* fac, int nn, int nz, int K, double fxabs, double dz, int which,
-                       double a, hipStream_t s);
+hipError_t fdz_factors(double2* fac, int nn, int nz, int K, int wpr, double fxabs, double dz,
+                       int which, double a, hipStream_t s);
 template <typename T>
 hipError_t fdz_set_dt(FdzJob<T>* jobs, int job, double dt, double lo_w, double lo_t,
                       hipStream_t s);
