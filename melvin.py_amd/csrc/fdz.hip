@@ -485,7 +485,7 @@ __device__ __forceinline__ void fdz_cols_body(const FdzJob<T>& J, int local, int
     // spread Tm's rounding over every mode.  Tm is constant along x, so
     // F_x(u Tm) = Tm F_x(u) exactly; the transforms carry T - Tm and the
     // Tm u_s term is added per mode after the forward transform.
-    __shared__ T tmean[kNT];
+    __shared__ T tmean[TC];  // (every size's body has its own)
     for (int cc = threadIdx.x; cc < ncol; cc += kNT) tmean[cc] = src[3][j0 + cc].x;
     const int nslot = 4 * npair;
     const int nzero = N - 2 * nn + 1;
