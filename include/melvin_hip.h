@@ -226,7 +226,7 @@ typedef struct mlv_job_desc {
                           state (value = state - residual) as bfloat16 (re, im)
                           pairs, 4 bytes per spectral element,
                           zero-initialised; or NULL */
-  void* work;          /* mlv_job_work_bytes() bytes */
+  void* work;          /* mlv_job_work_bytes() bytes, 16-byte aligned */
   void* psi;           /* optional spectral psi of the step, or NULL */
   void* phys[8];       /* optional physical outputs (problem order), or NULL */
   int curr;            /* ring slot the next step writes */

@@ -821,6 +821,8 @@ int build_job(const mlv_job_desc& d, JobDev<T>& J, int j) {
   if (!(d.lx > 0) || !(d.lz > 0) || !(d.dt > 0)) return bad("lengths and dt must be positive");
   if (d.curr < 0 || d.curr >= d.ab_order || (d.parity & ~1)) return bad("curr/parity out of range");
   if (!d.work) return bad("work is NULL");
+  // the z pass loads and the x-inverse stores row pairs as 16-byte vectors
+  if (reinterpret_cast<uintptr_t>(d.work) % 16) return bad("work must be 16-byte aligned");
   for (int f = 0; f < nf; ++f)
     if (!d.state[0][f] || !d.state[1][f] || !d.hist[f]) return bad("state/hist pointer is NULL");
   J.problem = d.problem;

@@ -121,14 +121,32 @@ __device__ __forceinline__ void stepA_body(const JobDev<T>& J, const StageDev& S
   else
     __syncthreads();
   fft_lds<T, LOGN, +1, NT, P>(lds, ncol, tw);
-  // x % kXil fastest, then the column: consecutive lanes store one row
-  // group's TC columns, contiguous in the interleaved layout
+  if constexpr (sizeof(T) == 4 && kXil == 2) {
+    // a lane stores both rows of a row pair of one column, side by side in
+    // the interleaved layout, as one 16-byte store; consecutive lanes take
+    // consecutive columns (the channel base and the row pairs are 16-byte
+    // aligned: work_chs and x nm are even)
+    static_assert(ITER % 2 == 0, "row pairs");
 #pragma unroll 8
-  for (int i = 0; i < ITER; ++i) {
-    const int e = threadIdx.x + i * NT;
-    const int g = e / kXil;
-    const int c = g % TC, x = (g / TC) * kXil + e % kXil;
-    if (c < ncol) dst[xrow_base(x, nm) + (size_t)(c0 + c) * kXil] = lds[c * LDSN + fft_pad(x)];
+    for (int i = 0; i < ITER / 2; ++i) {
+      const int g = threadIdx.x + i * NT;
+      const int c = g % TC, x = (g / TC) * 2;
+      if (c < ncol) {
+        const cx<T> a = lds[c * LDSN + fft_pad(x)], b = lds[c * LDSN + fft_pad(x + 1)];
+        *reinterpret_cast<float4*>(dst + xrow_base(x, nm) + (size_t)(c0 + c) * 2) =
+            make_float4(a.x, a.y, b.x, b.y);
+      }
+    }
+  } else {
+    // x % kXil fastest, then the column: consecutive lanes store one row
+    // group's TC columns, contiguous in the interleaved layout
+#pragma unroll 8
+    for (int i = 0; i < ITER; ++i) {
+      const int e = threadIdx.x + i * NT;
+      const int g = e / kXil;
+      const int c = g % TC, x = (g / TC) * kXil + e % kXil;
+      if (c < ncol) dst[xrow_base(x, nm) + (size_t)(c0 + c) * kXil] = lds[c * LDSN + fft_pad(x)];
+    }
   }
 }
 
