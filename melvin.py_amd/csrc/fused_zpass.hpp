@@ -232,7 +232,10 @@ __device__ __forceinline__ void stepB_body(const JobDev<T>& J, const StageDev& S
     // slot s is loaded by wavefront s and transformed by the same
     // wavefront: no workgroup barrier before the inverse transform
     constexpr bool WL_IN = fft_tps<N, P>() == 64 && Sh::WPS == 1 && Sh::SL == Sh::NW;
+    // a wavefront issuing its loads goes ahead of other blocks' arithmetic
+    __builtin_amdgcn_s_setprio(1);
     bload<T, Sh>(glb(J.work), chs, nm, x0, nseq, nslot, lds);
+    __builtin_amdgcn_s_setprio(0);
     if constexpr (WL_IN)
       wave_sync();
     else
