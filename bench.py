@@ -178,6 +178,25 @@ def ddc_leg(m, args):
     return res
 
 
+def fdz_measured_traffic(nx, nz):
+    """HBM bytes per launch of the FD-z column and row kernels of one FP32
+    job, from FETCH_SIZE / WRITE_SIZE calibrated against a plain copy, in
+    the committed counter summary; None when it does not cover nx x nz."""
+    path = os.path.join(ROOT, "profiles", f"rbc{nx}x{nz}_fdz_counters.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    block = text.split("```json", 1)[-1].split("```", 1)[0]
+    try:
+        d = json.loads(block)
+        if (d["nx"], d["nz"]) != (nx, nz):
+            return None
+        return d["bytes_per_launch"]
+    except (ValueError, KeyError):
+        return None
+
+
 def fdz_roofline(nx, nz, ms_step, kernel_ms, cbytes=8):
     """HBM roofline of the FD-z step (one Rayleigh-Benard job, FP32 state,
     FP64 tridiagonal factors).  Algorithmic bytes of one evaluation of the
@@ -204,6 +223,13 @@ def fdz_roofline(nx, nz, ms_step, kernel_ms, cbytes=8):
         for name, b, ms in (("x_transforms", cols, kernel_ms[0]), ("z_rows", rows, kernel_ms[1])):
             out["per_kernel"][name] = {"bytes_per_launch": b, "ms": ms,
                                        "GB/s": b / (ms * 1e-3) / 1e9}
+        # measured HBM bytes per step: two evaluations, each one launch of
+        # either kernel
+        meas = fdz_measured_traffic(nx, nz)
+        if meas:
+            out["traffic"] = 2 * (meas["x_transforms"] + meas["z_rows"])
+            for name in ("x_transforms", "z_rows"):
+                out["per_kernel"][name]["traffic_bytes_per_launch"] = meas[name]
     return out
 
 
